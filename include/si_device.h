@@ -1,0 +1,194 @@
+/*
+ * si_device.h -- device-resident superset index and batched overlap queries
+ * on AMD Instinct MI355X (gfx950).
+ *
+ * This is the batch boundary of libsuperintervals_hip.so.  The reference
+ * (kcleal/superintervals v1.0.0) answers one query per call; its only batch
+ * surface is the Python loop in src/superintervals/intervalmap.pyx:363-494
+ * (count_batch / search_idxs_batch / search_values_batch).  The entry points
+ * below take whole batches of queries, produce per-query results identical to
+ * the reference's single-query functions, and are the calls the C++ template
+ * (superintervals.hpp), the C-ABI (c_superintervals.h) and the Python package
+ * are built on.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  "stream" is a hipStream_t passed as void*
+ *    (NULL = the default stream).  Every query entry point is stream-ordered
+ *    and re-entrant: concurrent calls on one built index from several streams
+ *    or threads are safe (the index is read-only after build).
+ *  - Coordinates are int32_t or int64_t, chosen by coord_bytes (4 or 8) when
+ *    the index is created; query arrays use the same type.
+ *  - Positions ("idx") are uint32_t positions in the sorted order; the maximum
+ *    index size is 2^32 - 2 intervals.  SI_INDEX_NONE marks "no interval".
+ *  - Functions named *Host take host buffers and synchronise; all others take
+ *    device buffers (hipMalloc'd or torch tensors) and do not synchronise.
+ *  - Every function returns SI_OK (0) or an SI_ERR_* code; siGetLastError()
+ *    returns a message for the calling thread's last failure.
+ */
+#ifndef SI_DEVICE_H
+#define SI_DEVICE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SI_OK 0
+#define SI_ERR_ARG 1          /* invalid argument (null pointer, bad size, bad mode) */
+#define SI_ERR_HIP 2          /* a HIP runtime call or kernel launch failed */
+#define SI_ERR_NOT_BUILT 3    /* query on an index that has not been built */
+#define SI_ERR_CAPACITY 4     /* output capacity smaller than the number of results */
+#define SI_ERR_NOMEM 5        /* device or host allocation failed */
+#define SI_ERR_UNSUPPORTED 6  /* operation not available for this coordinate type */
+
+/* Output order of index lists. */
+#define SI_ORDER_DESC 0   /* strict descending position: c_superintervals.h:610-643 searchIdxs,
+                             superintervals.hpp:153-235 IndexIterator, search_values order */
+#define SI_ORDER_VECTOR 1 /* superintervals.hpp:879-905 search_idxs(s, e, std::vector&):
+                             the leading contiguous run [i*+1 .. idx] ascending, then the
+                             branch walk's hits descending */
+
+#define SI_INDEX_NONE 0xFFFFFFFFu
+
+typedef struct siIndex siIndex;
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char* siGetLastError(void);
+
+/* Library version string. */
+const char* siVersion(void);
+
+/* ---- Index lifecycle ------------------------------------------------------ */
+
+/* Create an empty index for coordinates of coord_bytes (4 = int32, 8 = int64).
+ * Returns NULL on failure. */
+siIndex* siIndexCreate(int coord_bytes);
+
+/* Free the index and its device memory. */
+void siIndexDestroy(siIndex* ix);
+
+/* Build the index from n intervals (end-inclusive), replacing any previous
+ * contents.  Replaces superintervals.hpp:112-130 (build: sort_intervals
+ * :1415-1439 then the branch stack) and c_superintervals.h:487-521
+ * (indexSuperIntervals).
+ *   starts, ends   n coordinates each, host memory if on_device == 0, else device.
+ *   values         optional int32 payload (NULL: none); kept in sorted order on
+ *                  the device for siSearchValuesFill.
+ *   start_sorted, end_sorted  the reference's add() flags (superintervals.hpp:95-105):
+ *                  start_sorted != 0 promises non-decreasing starts; end_sorted != 0
+ *                  additionally promises non-increasing ends within equal starts.
+ *                  When both hold the sort is skipped; otherwise a stable device sort
+ *                  by (start ascending, end descending) runs, so equal (start, end)
+ *                  pairs keep their input order (the C reference's qsort order).
+ * Synchronises the stream. */
+int siIndexBuild(siIndex* ix, const void* starts, const void* ends, const int32_t* values,
+                 size_t n, int start_sorted, int end_sorted, int on_device, void* stream);
+
+/* Number of intervals in the built index (0 before the first build). */
+size_t siIndexSize(const siIndex* ix);
+
+/* Coordinate width in bytes (4 or 8). */
+int siIndexCoordBytes(const siIndex* ix);
+
+/* Copy the built arrays to host buffers (each may be NULL):
+ *   starts, ends  sorted coordinates (n each)
+ *   values        int32 payload in sorted order (index built with values only)
+ *   perm          perm[k] = input position of the interval at sorted position k
+ *   branch        superset index as 64-bit values, UINT64_MAX = none (the
+ *                 reference's SIZE_MAX), ready to store into a size_t array. */
+int siIndexDownload(const siIndex* ix, void* starts, void* ends, int32_t* values, uint32_t* perm,
+                    uint64_t* branch);
+
+/* Device pointers of the built arrays (valid until the next build/destroy). */
+const void* siIndexDeviceStarts(const siIndex* ix);
+const void* siIndexDeviceEnds(const siIndex* ix);
+const uint32_t* siIndexDeviceBranch(const siIndex* ix);
+const uint32_t* siIndexDevicePerm(const siIndex* ix);
+
+/* Milliseconds spent in the last siIndexBuild on the device (sort, branch, trees). */
+double siIndexLastBuildMs(const siIndex* ix);
+
+/* ---- Batched queries on device buffers ------------------------------------ */
+
+/* upper_bound for n values: out[k] = last position with starts[pos] <= xs[k], or
+ * SI_INDEX_NONE.  Replaces superintervals.hpp:501-516 / c_superintervals.h:537-568. */
+int siUpperBound(const siIndex* ix, const void* xs, size_t n, uint32_t* out, void* stream);
+
+/* Overlap counts: counts[q] = #{j : starts[j] <= qe[q] && ends[j] >= qs[q]}, computed by
+ * the superset walk.  Replaces superintervals.hpp:651-825 (count) and
+ * c_superintervals.h:729-756 (countOverlaps). */
+int siCount(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint32_t* counts,
+            void* stream);
+
+/* has_overlaps / anyOverlaps (superintervals.hpp:865-871, c_superintervals.h:570-573). */
+int siHasOverlaps(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint8_t* out,
+                  void* stream);
+
+/* Workspace bytes siSearchOffsets needs for nq queries. */
+size_t siSearchWorkspaceBytes(size_t nq);
+
+/* First pass of search_idxs: offsets[0..nq] (nq + 1 entries) = exclusive prefix sum of
+ * the per-query overlap counts; offsets[nq] is the total.  workspace may be NULL (the
+ * library then allocates stream-ordered memory) or point to siSearchWorkspaceBytes(nq)
+ * device bytes.  counts may be NULL or receive the per-query counts. */
+int siSearchOffsets(const siIndex* ix, const void* qs, const void* qe, size_t nq,
+                    uint32_t* counts, uint64_t* offsets, void* workspace, size_t workspace_bytes,
+                    void* stream);
+
+/* Second pass: write each query's overlap positions at idx[offsets[q] .. offsets[q+1]) in
+ * the given order (SI_ORDER_DESC or SI_ORDER_VECTOR).  Replaces superintervals.hpp:879-905
+ * (search_idxs), :153-235 (IndexIterator) and c_superintervals.h:610-643 (searchIdxs). */
+int siSearchFill(const siIndex* ix, const void* qs, const void* qe, size_t nq, int order,
+                 const uint64_t* offsets, uint32_t* idx, void* stream);
+
+/* As siSearchFill but writes the int32 payload given at build time, in strict
+ * descending order.  Replaces superintervals.hpp:551-579 (search_values) and
+ * c_superintervals.h:575-608 (searchValues). */
+int siSearchValuesFill(const siIndex* ix, const void* qs, const void* qe, size_t nq,
+                       const uint64_t* offsets, int32_t* values, void* stream);
+
+/* coverage for int32 indexes: count_out[q] and cov_out[q] = sum over overlaps of
+ * min(ends, qe) - max(starts, qs) in int32 arithmetic (no +1), as
+ * c_superintervals.h:758-792. */
+int siCoverage(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint32_t* count_out,
+               int32_t* cov_out, void* stream);
+
+/* Reference-walk statistics summed over the batch: totals[0] = hits, totals[1] = reads of
+ * ends[] by the walk, totals[2] = reads of branch[] (jumps), totals[3] = queries whose
+ * upper_bound found a position.  The walk is c_superintervals.h:610-643. */
+int siWalkStats(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint64_t* totals,
+                void* stream);
+
+/* ---- Host-buffer convenience forms (copy in, compute, copy out, synchronise) -- */
+
+int siCountHost(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint32_t* counts);
+
+/* Two-pass search into host CSR buffers.  offsets has nq + 1 entries; idx has capacity
+ * cap.  *total receives the number of results; SI_ERR_CAPACITY if it exceeds cap (offsets
+ * are still written).  values != NULL selects payload output instead of positions
+ * (then order must be SI_ORDER_DESC and idx may be NULL). */
+int siSearchHost(const siIndex* ix, const void* qs, const void* qe, size_t nq, int order,
+                 uint64_t* offsets, uint32_t* idx, int32_t* values, size_t cap, uint64_t* total);
+
+int siUpperBoundHost(const siIndex* ix, const void* xs, size_t n, uint32_t* out);
+
+int siHasOverlapsHost(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint8_t* out);
+
+int siCoverageHost(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint32_t* count_out,
+                   int32_t* cov_out);
+
+/* ---- Synthetic workloads --------------------------------------------------- */
+
+/* Deterministic generators for the benchmark workloads 1-4 (see workload.cpp).
+ * starts/ends are host buffers of n coordinates of coord_bytes each. */
+int siWorkloadIntervals(int config, uint64_t seed, size_t n, int coord_bytes, int sorted,
+                        void* starts, void* ends);
+int siWorkloadQueries(int config, uint64_t seed, size_t n, int coord_bytes, void* qs, void* qe);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SI_DEVICE_H */

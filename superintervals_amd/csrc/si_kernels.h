@@ -1,0 +1,70 @@
+// si_kernels.h -- device kernels of the superset index (gfx950, wave64).
+//
+// Layout in HBM of a built index of n intervals (coordinate type S):
+//   starts[n], ends[n]   S, sorted by (start asc, end desc)
+//   branch[n]            uint32, nearest j < i with ends[j] >= ends[i], or NONE
+//   perm[n]              uint32, input position of the interval at sorted position k
+//   values[n]            int32 payload in sorted order (optional)
+//   tree levels          S, a 64-ary max tree over ends: level l+1 holds the max of
+//                        each group of 64 entries of level l (level 0 is ends)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sik {
+
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr int WAVE = 64;
+constexpr int FAN = 64;          // max-tree fan-out (one wave-wide load per node)
+constexpr int MAX_LEVELS = 8;
+
+template <typename S>
+struct TreeDesc {
+    const S* lvl[MAX_LEVELS];    // lvl[0] = ends
+    uint32_t size[MAX_LEVELS];
+    int nlev;
+};
+
+template <typename S>
+struct IndexView {
+    const S* __restrict__ starts;
+    const S* __restrict__ ends;
+    const uint32_t* __restrict__ branch;
+    uint32_t n;
+};
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// ---- upper_bound ------------------------------------------------------------
+// Last position with starts[pos] <= x, or NONE (superintervals.hpp:501-516
+// semantics; written as the branchless "base/len" halving).
+template <typename S>
+__device__ __forceinline__ uint32_t upper_bound_pos(const S* __restrict__ st, uint32_t n, S x) {
+    if (n == 0) return NONE;
+    const S* base = st;
+    uint32_t len = n;
+    while (len > 1) {
+        uint32_t half = len >> 1;
+        base = (base[half] <= x) ? base + half : base;
+        len -= half;
+    }
+    uint32_t c = (uint32_t)(base - st) + ((*base <= x) ? 1u : 0u);
+    return c - 1u;   // c == 0 wraps to NONE
+}
+
+// ---- the backward superset walk ---------------------------------------------
+// From position i (upper_bound of the query end) walk backwards: a hit (s <= ends[i])
+// steps to i - 1, a miss jumps to branch[i].  This single loop is the two-phase walk
+// of c_superintervals.h:618-642 (leading run, then branch jumps).
+template <typename S>
+__device__ __forceinline__ uint32_t walk_count(const IndexView<S>& ix, uint32_t i, S s) {
+    uint32_t c = 0;
+    while (i != NONE) {
+        if (s <= ix.ends[i]) { ++c; --i; }
+        else i = ix.branch[i];
+    }
+    return c;
+}
+
+}  // namespace sik
