@@ -1,0 +1,175 @@
+"""Benchmark: batched overlap search on one MI355X.
+
+Workload (log-normal intervals, "config 2"): 10M int32 intervals, starts uniform on
+[0, 2.5e8), lengths clamp(round(LogNormal(6, 1.5)), 1, 1e6); 10M queries, lengths
+max(1, round(LogNormal(7, 1))).  Inputs are synthetic (seeded generator) and resident
+in HBM before timing.  One step = search_idxs for every query in two passes:
+count -> exclusive scan (offsets) -> fill of the CSR index lists (descending order).
+
+Prints one JSON line (the metric contract) with the roofline of the dominant kernel
+and a CPU baseline (the reference's own code, compiled from its headers into
+oracle/_ref when available, else the oracle's restatement) on a bounded query sample.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--n", type=int, default=10_000_000)
+    p.add_argument("--nq", type=int, default=10_000_000)
+    p.add_argument("--cpu-sample", type=int, default=2_000_000)
+    p.add_argument("--no-cpu", action="store_true")
+    return p.parse_args()
+
+
+def cpu_baseline(s, e, qs, qe, sample):
+    """Reference CPU path on a bounded sample of the same queries, all host cores."""
+    import numpy as np
+    import oracle
+    threads = len(os.sched_getaffinity(0))
+    sq, se = qs[:sample], qe[:sample]
+    if oracle.Reference.available():
+        R = oracle.Reference()
+        m = oracle.RefMap(R, s, e)
+        m.search_batch_total(sq[:10000], se[:10000], threads)        # warm-up
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            m.search_batch_total(sq, se, threads)
+            ts.append(time.perf_counter() - t0)
+        kind = "reference"
+        desc = "si::IntervalMap<int,int>::search_idxs(s, e, std::vector&) from the reference headers"
+    else:
+        O = oracle.Oracle()
+        ix = O.build_index(s, e)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            O.search_batch(ix, sq, se, mode=1, threads=threads, lists=False)
+            ts.append(time.perf_counter() - t0)
+        kind = "port"
+        desc = "oracle restatement of search_idxs"
+    t = sorted(ts)[1]
+    return {"value": sample / t, "unit": "queries/s", "cores": threads, "kind": kind,
+            "sample": f"first {sample} of the {qs.size} queries, median of 3; {desc}; "
+                      f"static chunks over {threads} threads"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import superintervals_amd as sia
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    s, e = sia.workload_intervals(2, a.n, seed=1, sorted=True)
+    qs, qe = sia.workload_queries(2, a.nq, seed=2)
+    ix = sia.DeviceIndex(np.int32).build(s, e)
+    build_ms = ix.last_build_ms
+    tqs, tqe = torch.from_numpy(qs).to(dev), torch.from_numpy(qe).to(dev)
+    nq = a.nq
+    off = torch.empty(nq + 1, dtype=torch.int64, device=dev)
+    ws = torch.empty(sia.DeviceIndex.workspace_bytes(nq) // 8 + 1, dtype=torch.int64, device=dev)
+    ix.offsets_device(tqs, tqe, off, None, ws)
+    torch.cuda.synchronize()
+    total = int(off[-1].item())
+    idx = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+    counts = torch.empty(nq, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    # reference-walk statistics for the algorithmic byte model
+    hits, V, J, found = ix.walk_stats_device(tqs, tqe)
+
+    def step(evs=None):
+        if evs:
+            evs[0].record(stream)
+        ix.offsets_device(tqs, tqe, off, None, ws, stream=stream)
+        if evs:
+            evs[1].record(stream)
+        ix.fill_device(tqs, tqe, off, idx, order="desc", stream=stream)
+        if evs:
+            evs[2].record(stream)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(a.steps)]
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_off = sum(ev[0].elapsed_time(ev[1]) for ev in evs) / a.steps
+    t_fill = sum(ev[1].elapsed_time(ev[2]) for ev in evs) / a.steps
+    ms_step = wall * 1e3 / a.steps
+
+    # count-only kernel, timed alone
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for _ in range(2):
+        ix.count_device(tqs, tqe, counts, stream=stream)
+    c0.record(stream)
+    for _ in range(a.steps):
+        ix.count_device(tqs, tqe, counts, stream=stream)
+    c1.record(stream)
+    torch.cuda.synchronize()
+    t_count = c0.elapsed_time(c1) / a.steps
+
+    c = 4
+    logn = math.ceil(math.log2(a.n))
+    b_count = nq * (2 * c + 4) + found * c * logn + V * c + 4 * J     # bytes per count pass
+    b_search = b_count + 4 * hits + 8 * nq
+    kernels = {
+        "count": {"ms": t_count, "alg_bytes": b_count, "GBps": b_count / t_count / 1e6},
+        "search_offsets": {"ms": t_off},
+        "search_fill": {"ms": t_fill},
+        "search_total": {"ms": t_off + t_fill, "alg_bytes": b_search,
+                         "GBps": b_search / (t_off + t_fill) / 1e6},
+    }
+    dom = "search_fill" if t_fill >= t_off else "search_offsets"
+    ach = b_count / t_count / 1e6
+    roof = {"bound": "hbm", "kernel": "k_count (count pass, one launch over all queries)",
+            "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+
+    out = {
+        "metric": "overlap queries/sec + achieved HBM GB/s (10M idx x 10M qry); p50/p99 batch latency",
+        "value": nq / (ms_step / 1e3),
+        "unit": "queries/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic (seeded log-normal generator, inputs resident in HBM)",
+        "config": {"workload": "search_idxs two-pass, 10M int32 intervals x 10M queries (log-normal lengths)",
+                   "intervals": a.n, "queries": nq, "hits": total, "order": "descending",
+                   "build_ms": round(build_ms, 2), "dominant_search_kernel": dom},
+        "roofline": roof,
+        "kernels": kernels,
+        "count_queries_per_s": nq / (t_count / 1e3),
+        "walk": {"hits": hits, "ends_reads": V, "branch_reads": J, "found": found},
+    }
+    if not a.no_cpu:
+        cb = cpu_baseline(s, e, qs, qe, min(a.cpu_sample, nq))
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = out["value"] / cb["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -56,6 +56,22 @@ uint64_t or_hash_list_u32(const uint32_t* v, size_t n) {
     return h;
 }
 
+/* Per-query hashes of a CSR result (offsets[nq+1], idx as uint32 or int64). */
+void or_hash_csr_u32(size_t nq, const uint64_t* off, const uint32_t* idx, uint64_t* out) {
+    for (size_t q = 0; q < nq; ++q) out[q] = or_hash_list_u32(idx + off[q], (size_t)(off[q + 1] - off[q]));
+}
+void or_hash_csr_i64(size_t nq, const uint64_t* off, const int64_t* idx, uint64_t* out) {
+    for (size_t q = 0; q < nq; ++q) out[q] = or_hash_list(idx + off[q], (size_t)(off[q + 1] - off[q]));
+}
+void or_hash_csr_i32(size_t nq, const uint64_t* off, const int32_t* v, uint64_t* out) {
+    for (size_t q = 0; q < nq; ++q) {
+        uint64_t h = or_mix64((uint64_t)(off[q + 1] - off[q]) + 0x51ED270B27B1ULL);
+        for (uint64_t k = off[q]; k < off[q + 1]; ++k)
+            h = or_mix64(h ^ ((uint64_t)(int64_t)v[k] + 0x9E3779B97F4A7C15ULL));
+        out[q] = h;
+    }
+}
+
 static int or_nthreads_clamp(int t) { return t < 1 ? 1 : (t > 256 ? 256 : t); }
 
 #define OR_DEFINE(TY, SFX)                                                          \

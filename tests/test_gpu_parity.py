@@ -1,0 +1,325 @@
+"""GPU parity: the HIP library against the golden vectors and the CPU oracle.
+
+Everything here calls libsuperintervals_hip.so through its C-ABI (ctypes) and
+needs a GPU.  Integer results must be bit-exact: same sorted arrays, same branch,
+same counts, same index lists in the same order.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_CASES, load_golden, sha
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sia():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import superintervals_amd
+    return superintervals_amd
+
+
+def _dev_index(sia, s, e, values=None):
+    ix = sia.DeviceIndex(s.dtype)
+    ix.build(s, e, values)
+    return ix
+
+
+def _hash(orc, off, idx):
+    return orc.hash_csr(off, idx)
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_golden_build_and_queries(sia, orc, name):
+    print(f"[gpu] golden {name}", flush=True)
+    rec, s, e, qs, qe = load_golden(name)
+    data = np.arange(s.size, dtype=np.int32)
+    ix = _dev_index(sia, s, e, data)
+    arr = ix.download(values=True)
+    assert sha(arr["starts"], arr["ends"]) == str(rec["sorted_sha256"])
+    assert sha(arr["branch"]) == str(rec["branch_sha256"])
+    if "c_data_sha256" in rec:
+        assert sha(arr["values"]) == str(rec["c_data_sha256"])
+        assert sha(arr["perm"].astype(np.int32)) == str(rec["c_data_sha256"])
+    counts = ix.count(qs, qe)
+    assert np.array_equal(counts, rec["counts"])
+    off, idx = ix.search(qs, qe, order="desc")
+    assert np.array_equal(np.diff(off), rec["counts"].astype(np.uint64))
+    hd = orc.hash_csr(off, idx)
+    assert np.array_equal(hd[:rec["hash_desc_prefix"].size], rec["hash_desc_prefix"])
+    assert orc.digest(hd) == int(rec["hash_desc_digest"])
+    offv, idxv = ix.search(qs, qe, order="vector")
+    assert np.array_equal(offv, off)
+    hv = orc.hash_csr(offv, idxv)
+    assert orc.digest(hv) == int(rec["hash_vec_digest"])
+    lo = rec["lists_offsets"]
+    for q in range(lo.size - 1):
+        assert np.array_equal(idx[off[q]:off[q + 1]].astype(np.int64), rec["lists_idx"][lo[q]:lo[q + 1]])
+    k = rec["upper_bound"].size
+    ub = ix.upper_bound(qe[:k]).astype(np.int64)
+    ub[ub == 0xFFFFFFFF] = -1
+    assert np.array_equal(ub, rec["upper_bound"])
+    assert np.array_equal(np.packbits(ix.has_overlaps(qs, qe)), rec["has_overlaps"])
+    if "hash_cvalues_digest" in rec:
+        offval, vals = ix.search(qs, qe, values=True)
+        assert np.array_equal(offval, off)
+        assert orc.digest(orc.hash_csr(offval, vals)) == int(rec["hash_cvalues_digest"])
+        kc = rec["coverage_prefix"].shape[0]
+        c, v = ix.coverage(qs[:kc], qe[:kc])
+        assert np.array_equal(np.stack([c.astype(np.int64), v.astype(np.int64)], 1), rec["coverage_prefix"])
+    ix.close()
+
+
+@pytest.mark.parametrize("seed", range(8))
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_random_vs_oracle(sia, orc, seed, dtype):
+    print(f"[gpu] random seed={seed} dtype={np.dtype(dtype).name}", flush=True)
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.choice([1, 2, 63, 64, 65, 4095, 4096, 4097, 20_000, 150_000]))
+    span = int(rng.choice([50, 5_000, 1_000_000]))
+    s = rng.integers(-span, span, n).astype(dtype)
+    L = rng.geometric(1.0 / max(2, span // 50), n)
+    e = (s + L - 1).astype(dtype)
+    if seed % 2:   # presorted input takes the no-sort path
+        o = np.lexsort((-e.astype(np.int64), s.astype(np.int64)))
+        s, e = s[o], e[o]
+    if dtype == np.int64:
+        s = s + (dtype(3) << 40)
+        e = e + (dtype(3) << 40)
+    nq = 30_000
+    qs = rng.integers(-span - 10, span + 10, nq).astype(dtype)
+    qe = (qs + rng.integers(-5, span // 20 + 5, nq)).astype(dtype)   # a few inverted
+    if dtype == np.int64:
+        qs = qs + (dtype(3) << 40)
+        qe = qe + (dtype(3) << 40)
+    oix = orc.build_index(s, e)
+    ix = _dev_index(sia, s, e, np.arange(n, dtype=np.int32))
+    a = ix.download(values=True)
+    assert np.array_equal(a["starts"], oix["starts"]) and np.array_equal(a["ends"], oix["ends"])
+    assert np.array_equal(a["branch"], oix["branch"])
+    assert np.array_equal(a["values"], oix["data"])
+    assert np.array_equal(ix.count(qs, qe), orc.count_batch(oix, qs, qe))
+    for order, mode in (("desc", 0), ("vector", 1)):
+        off, idx = ix.search(qs, qe, order=order)
+        ooff, oidx = orc.search_batch(oix, qs, qe, mode=mode)
+        assert np.array_equal(off, ooff)
+        assert np.array_equal(idx.astype(np.int64), oidx)
+    off, vals = ix.search(qs, qe, values=True)
+    assert np.array_equal(vals, oix["data"][orc.search_batch(oix, qs, qe, mode=0)[1]])
+    ub = ix.upper_bound(qe).astype(np.int64)
+    ub[ub == 0xFFFFFFFF] = -1
+    assert np.array_equal(ub, np.searchsorted(oix["starts"], qe, side="right") - 1)
+    ho = np.array([orc.has_overlaps(oix, x, y) for x, y in zip(qs[:3000], qe[:3000])])
+    assert np.array_equal(ix.has_overlaps(qs[:3000], qe[:3000]), ho)
+    if dtype == np.int32:
+        c, v = ix.coverage(qs[:3000], qe[:3000])
+        want = [orc.coverage(oix, x, y) for x, y in zip(qs[:3000], qe[:3000])]
+        assert np.array_equal(np.stack([c, v], 1).astype(np.int64), np.array(want, np.int64))
+    ix.close()
+
+
+def test_walk_stats_match_oracle(sia, orc):
+    import torch
+    rec, s, e, qs, qe = load_golden("c1_sorted")
+    ix = _dev_index(sia, s, e)
+    tot = ix.walk_stats_device(torch.from_numpy(qs).cuda(), torch.from_numpy(qe).cuda())
+    oix = orc.build_index(s, e)
+    h, v, j = orc.walk_stats_batch(oix, qs, qe)
+    assert tot[:3] == [h, v, j]
+    assert tot[0] == int(rec["counts"].sum())
+
+
+def test_empty_and_edge_cases(sia, orc):
+    # empty index: every query is well defined and finds nothing
+    ix = sia.DeviceIndex(np.int32).build(np.zeros(0, np.int32), np.zeros(0, np.int32))
+    q = np.array([1, 5], np.int32)
+    assert ix.count(q, q).tolist() == [0, 0]
+    off, idx = ix.search(q, q)
+    assert off.tolist() == [0, 0, 0] and idx.size == 0
+    assert ix.has_overlaps(q, q).tolist() == [False, False]
+    # empty query batch
+    ix1 = sia.DeviceIndex(np.int32).build(np.array([1], np.int32), np.array([10], np.int32))
+    assert ix1.count(np.zeros(0, np.int32), np.zeros(0, np.int32)).size == 0
+    off, idx = ix1.search(np.zeros(0, np.int32), np.zeros(0, np.int32))
+    assert off.tolist() == [0] and idx.size == 0
+    assert ix1.count([1, 11], [5, 20]).tolist() == [1, 0]
+    # int32 extremes
+    big, small = np.iinfo(np.int32).max, np.iinfo(np.int32).min
+    s = np.array([small, small, 0, big - 5, big], np.int32)
+    e = np.array([small, big, 10, big, big], np.int32)
+    qs = np.array([small, small, 5, big, big, 0, big], np.int32)
+    qe = np.array([small, big, 5, big, big - 1, small, small], np.int32)
+    ixe = sia.DeviceIndex(np.int32).build(s, e)
+    oix = orc.build_index(s, e)
+    assert np.array_equal(ixe.count(qs, qe), orc.count_batch(oix, qs, qe))
+    # query before build
+    with pytest.raises(sia.SIError):
+        sia.DeviceIndex(np.int32).count(q, q)
+
+
+def _capi():
+    import superintervals_amd._native as N
+    return N
+
+
+def test_capi_reference_functions(sia, orc):
+    """The reference C-ABI (c_superintervals.h names) on the device index."""
+    N = _capi()
+    L = N.lib
+    rec, s, e, qs, qe = load_golden("c1_shuffled")
+    h = L.createSuperIntervals()
+    L.reserveSuperIntervals(h, 16)
+    for i in range(s.size):
+        L.addInterval(h, int(s[i]), int(e[i]), i)
+    st = N.cSuperIntervals.from_address(h)
+    assert st.size == s.size and not st.startSorted
+    L.indexSuperIntervals(h)
+    st = N.cSuperIntervals.from_address(h)
+    assert st.startSorted and st.endSorted and st.idx == 0
+    n = s.size
+    hs = np.ctypeslib.as_array(st.starts, (n,)).copy()
+    he = np.ctypeslib.as_array(st.ends, (n,)).copy()
+    hd = np.ctypeslib.as_array(st.data, (n,)).copy()
+    hb = np.ctypeslib.as_array(st.branch, (n,)).copy().view(np.int64)
+    assert sha(hs, he) == str(rec["sorted_sha256"])
+    assert sha(hb) == str(rec["branch_sha256"])
+    assert sha(hd) == str(rec["c_data_sha256"])
+    oix = orc.build_index(s, e)
+    res = N.cIndexResult()
+    for q in range(0, 300):
+        a, b = int(qs[q]), int(qe[q])
+        assert L.countOverlaps(h, a, b) == rec["counts"][q]
+        L.clearIndexResult(C.byref(res))
+        L.searchIdxs(h, a, b, C.byref(res))
+        got = [res.data[k] for k in range(res.size)]
+        assert got == orc.search(oix, a, b, 0).tolist()
+        L.clearIndexResult(C.byref(res))
+        L.searchValues(h, a, b, C.byref(res))
+        assert [res.data[k] for k in range(res.size)] == orc.values(oix, a, b).tolist()
+        ub = L.upperBound(h, b)
+        want = orc.upper_bound(oix, b)
+        assert (ub == 2 ** 64 - 1 and want == -1) or ub == want
+        assert N.cSuperIntervals.from_address(h).idx == ub
+        assert L.anyOverlaps(h, a, b) == orc.has_overlaps(oix, a, b)
+        cnt = C.c_size_t(0)
+        cov = C.c_int32(0)
+        L.coverage(h, a, b, C.byref(cnt), C.byref(cov))
+        assert (cnt.value, cov.value) == orc.coverage(oix, a, b)
+    # appending semantics: results accumulate until cleared (c_superintervals.h:209-210)
+    L.clearIndexResult(C.byref(res))
+    L.searchIdxs(h, int(qs[0]), int(qe[0]), C.byref(res))
+    L.searchIdxs(h, int(qs[1]), int(qe[1]), C.byref(res))
+    assert res.size == rec["counts"][0] + rec["counts"][1]
+    L.destroyIndexResult(C.byref(res))
+    keys = N.cKeyResult()
+    items = N.cItemResult()
+    L.searchKeys(h, int(qs[5]), int(qe[5]), C.byref(keys))
+    L.searchItems(h, int(qs[5]), int(qe[5]), C.byref(items))
+    strict = orc.search(oix, qs[5], qe[5], 0)
+    assert [(keys.data[k].start, keys.data[k].end) for k in range(keys.size)] == \
+        [(int(oix["starts"][j]), int(oix["ends"][j])) for j in strict]
+    assert [items.data[k].data for k in range(items.size)] == [int(oix["data"][j]) for j in strict]
+    L.destroyKeyResult(C.byref(keys))
+    L.destroyItemResult(C.byref(items))
+    # batch extensions
+    counts = np.empty(qs.size, np.uint32)
+    assert L.siCountBatch(h, qs.ctypes.data, qe.ctypes.data, qs.size, counts.ctypes.data) == 0
+    assert np.array_equal(counts, rec["counts"])
+    off = np.empty(qs.size + 1, np.uint64)
+    tot = C.c_uint64(0)
+    rc = L.siSearchIdxsBatch(h, qs.ctypes.data, qe.ctypes.data, qs.size, off.ctypes.data, None, 0, C.byref(tot))
+    assert rc == N.SI_ERR_CAPACITY and tot.value == int(rec["counts"].sum())
+    idx = np.empty(tot.value, np.int32)
+    assert L.siSearchIdxsBatch(h, qs.ctypes.data, qe.ctypes.data, qs.size, off.ctypes.data, idx.ctypes.data,
+                               idx.size, C.byref(tot)) == 0
+    assert orc.digest(orc.hash_csr(off, idx.astype(np.int64))) == int(rec["hash_desc_digest"])
+    vals = np.empty(tot.value, np.int32)
+    assert L.siSearchValuesBatch(h, qs.ctypes.data, qe.ctypes.data, qs.size, off.ctypes.data,
+                                 vals.ctypes.data, vals.size, C.byref(tot)) == 0
+    assert orc.digest(orc.hash_csr(off, vals)) == int(rec["hash_cvalues_digest"])
+    L.destroySuperIntervals(h)
+
+
+def test_capi_empty_and_unindexed(sia):
+    N = _capi()
+    L = N.lib
+    h = L.createSuperIntervals()
+    L.indexSuperIntervals(h)              # empty: no-op, as the reference
+    assert L.countOverlaps(h, 1, 5) == 0
+    assert not L.anyOverlaps(h, 1, 5)
+    L.addInterval(h, 1, 10, 7)
+    assert L.countOverlaps(h, 1, 5) == 0   # not indexed yet: no result, error recorded
+    assert b"indexSuperIntervals" in L.siGetLastError()
+    L.indexSuperIntervals(h)
+    assert L.countOverlaps(h, 1, 5) == 1 and L.countOverlaps(h, 11, 20) == 0
+    L.destroySuperIntervals(h)
+
+
+def test_python_intervalmap_mirror(sia, orc):
+    """The Python IntervalMap mirrors intervalmap.pyx on the reference's examples."""
+    m = sia.IntervalMap()
+    for s, e, v in [(10, 20, "a"), (11, 12, None), (13, 14, None), (15, 16, None), (25, 29, "d")]:
+        m.add(s, e, v)
+    m.build()
+    assert m.has_overlaps(17, 30) and not m.has_overlaps(1, 3)
+    assert m.search_values(17, 30) == ["d", "a"]
+    assert m.count(17, 30) == 2
+    assert m.search_idxs(17, 30) == [4, 0]
+    assert m.coverage(10, 29)[1] == 17
+    m2 = sia.IntervalMap.from_arrays([1, 2, 4, 5, 6, 60], [100, 3, 50, 50, 50, 70])
+    assert m2.search_idxs(10, 65) == [2, 3, 4, 5, 0]          # vector order (pyx uses it)
+    assert [x.tolist() for x in m2.search_idxs_batch([10, 1], [65, 1])] == [[2, 3, 4, 5, 0], [0]]
+    assert m2.count_batch([10, 200], [65, 300]).tolist() == [5, 0]
+    with pytest.raises(ValueError):
+        m2.count_batch([1, 2], [3])
+
+
+def test_full_size_workload2_properties(sia, orc):
+    """10M intervals x 10M queries (workload 2): size-independent properties of the whole
+    output plus an oracle check of a query sample."""
+    import torch
+    print("[gpu] full-size workload 2", flush=True)
+    n = nq = 10_000_000
+    s, e = sia.workload_intervals(2, n, seed=1, sorted=True)
+    qs, qe = sia.workload_queries(2, nq, seed=2)
+    ix = _dev_index(sia, s, e)
+    dev = torch.device("cuda:0")
+    tqs, tqe = torch.from_numpy(qs).to(dev), torch.from_numpy(qe).to(dev)
+    off = torch.empty(nq + 1, dtype=torch.int64, device=dev)
+    counts = torch.empty(nq, dtype=torch.int32, device=dev)
+    ix.offsets_device(tqs, tqe, off, counts)
+    torch.cuda.synchronize()
+    total = int(off[-1].item())
+    # closed form: count = #starts <= e - #ends < s (all queries have s <= e)
+    ss = torch.from_numpy(s).to(dev)
+    se = torch.sort(torch.from_numpy(e).to(dev)).values
+    closed = torch.searchsorted(ss, tqe, right=True) - torch.searchsorted(se, tqs, right=False)
+    assert torch.equal(closed.to(torch.int32), counts)
+    assert torch.equal(off[1:] - off[:-1], counts.to(torch.int64))
+    idx = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+    ix.fill_device(tqs, tqe, off, idx, order="desc")
+    torch.cuda.synchronize()
+    d = ix.device_arrays()
+    ends_t = torch.from_numpy(ix.download()["ends"]).to(dev)
+    starts_t = ss
+    # every emitted position overlaps its query
+    qid = torch.repeat_interleave(torch.arange(nq, device=dev), counts.to(torch.int64))
+    pos = idx[:total].to(torch.int64)
+    assert bool((starts_t[pos] <= tqe[qid]).all()) and bool((ends_t[pos] >= tqs[qid]).all())
+    # strictly descending inside every list
+    same = qid[1:] == qid[:-1]
+    assert bool((pos[1:][same] < pos[:-1][same]).all())
+    # oracle on a sample
+    oix = orc.build_index(s, e)
+    sample = np.arange(0, nq, 4999)
+    ooff, oidx = orc.search_batch(oix, qs[sample], qe[sample], mode=0)
+    offh = off.cpu().numpy().astype(np.uint64)
+    idxh = idx[:total].cpu().numpy()
+    for k, q in enumerate(sample):
+        assert np.array_equal(idxh[offh[q]:offh[q + 1]].astype(np.int64), oidx[ooff[k]:ooff[k + 1]])
+    del d
+    ix.close()
