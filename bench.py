@@ -86,13 +86,14 @@ def main():
     tqs, tqe = torch.from_numpy(qs).to(dev), torch.from_numpy(qe).to(dev)
     nq = a.nq
     off = torch.empty(nq + 1, dtype=torch.int64, device=dev)
-    ws = torch.empty(sia.DeviceIndex.workspace_bytes(nq) // 8 + 1, dtype=torch.int64, device=dev)
-    ix.offsets_device(tqs, tqe, off, None, ws)
+    plan = torch.empty(ix.plan_bytes(nq), dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+    ix.plan_device(tqs, tqe, plan, stream=stream)
+    ix.offsets_planned(plan, nq, off, stream=stream)
     torch.cuda.synchronize()
     total = int(off[-1].item())
     idx = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
     counts = torch.empty(nq, dtype=torch.int32, device=dev)
-    stream = torch.cuda.current_stream()
 
     # reference-walk statistics for the algorithmic byte model
     hits, V, J, found = ix.walk_stats_device(tqs, tqe)
@@ -100,33 +101,37 @@ def main():
     def step(evs=None):
         if evs:
             evs[0].record(stream)
-        ix.offsets_device(tqs, tqe, off, None, ws, stream=stream)
+        ix.plan_device(tqs, tqe, plan, stream=stream)
         if evs:
             evs[1].record(stream)
-        ix.fill_device(tqs, tqe, off, idx, order="desc", stream=stream)
+        ix.offsets_planned(plan, nq, off, stream=stream)
         if evs:
             evs[2].record(stream)
+        ix.fill_planned(plan, nq, off, idx, order="desc", stream=stream)
+        if evs:
+            evs[3].record(stream)
 
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(a.steps)]
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(a.steps)]
     t0 = time.perf_counter()
     for k in range(a.steps):
         step(evs[k])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    t_off = sum(ev[0].elapsed_time(ev[1]) for ev in evs) / a.steps
-    t_fill = sum(ev[1].elapsed_time(ev[2]) for ev in evs) / a.steps
+    t_plan = sum(ev[0].elapsed_time(ev[1]) for ev in evs) / a.steps
+    t_off = sum(ev[1].elapsed_time(ev[2]) for ev in evs) / a.steps
+    t_fill = sum(ev[2].elapsed_time(ev[3]) for ev in evs) / a.steps
     ms_step = wall * 1e3 / a.steps
 
-    # count-only kernel, timed alone
+    # count kernel alone on the planned batch (one launch per pass)
     c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     for _ in range(2):
-        ix.count_device(tqs, tqe, counts, stream=stream)
+        ix.count_planned(plan, nq, counts, stream=stream)
     c0.record(stream)
     for _ in range(a.steps):
-        ix.count_device(tqs, tqe, counts, stream=stream)
+        ix.count_planned(plan, nq, counts, stream=stream)
     c1.record(stream)
     torch.cuda.synchronize()
     t_count = c0.elapsed_time(c1) / a.steps
@@ -137,14 +142,15 @@ def main():
     b_search = b_count + 4 * hits + 8 * nq
     kernels = {
         "count": {"ms": t_count, "alg_bytes": b_count, "GBps": b_count / t_count / 1e6},
+        "plan": {"ms": t_plan},
         "search_offsets": {"ms": t_off},
         "search_fill": {"ms": t_fill},
-        "search_total": {"ms": t_off + t_fill, "alg_bytes": b_search,
-                         "GBps": b_search / (t_off + t_fill) / 1e6},
+        "search_total": {"ms": t_plan + t_off + t_fill, "alg_bytes": b_search,
+                         "GBps": b_search / (t_plan + t_off + t_fill) / 1e6},
     }
-    dom = "search_fill" if t_fill >= t_off else "search_offsets"
+    dom = max(("plan", t_plan), ("search_offsets", t_off), ("search_fill", t_fill), key=lambda x: x[1])[0]
     ach = b_count / t_count / 1e6
-    roof = {"bound": "hbm", "kernel": "k_count (count pass, one launch over all queries)",
+    roof = {"bound": "hbm", "kernel": "k_walk<int, count> (count pass, one launch over all queries)",
             "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
 

@@ -110,7 +110,35 @@ const uint32_t* siIndexDevicePerm(const siIndex* ix);
 /* Milliseconds spent in the last siIndexBuild on the device (sort, branch, trees). */
 double siIndexLastBuildMs(const siIndex* ix);
 
-/* ---- Batched queries on device buffers ------------------------------------ */
+/* ---- Planned batches (device buffers) --------------------------------------
+ * A plan holds one batch of queries sorted by end together with each query's
+ * upper_bound position, so the count and fill kernels walk neighbouring queries in
+ * neighbouring lanes.  A plan is a caller-owned device buffer of siPlanBytes bytes;
+ * it is valid for the index and queries it was made from until the index is rebuilt.
+ * Calls that share one plan must be ordered on one stream (they reuse its scratch).
+ * siPlanQueries synchronises the stream once (radix pass selection). */
+size_t siPlanBytes(const siIndex* ix, size_t nq);
+
+int siPlanQueries(const siIndex* ix, const void* qs, const void* qe, size_t nq, void* plan,
+                  size_t plan_bytes, void* stream);
+
+/* counts[q] for the planned queries, in input order (see siCount). */
+int siCountPlanned(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, void* stream);
+
+/* offsets[0..nq] for the planned queries (see siSearchOffsets); counts may be NULL. */
+int siSearchOffsetsPlanned(const siIndex* ix, void* plan, size_t nq, uint32_t* counts,
+                           uint64_t* offsets, void* stream);
+
+/* Fill pass for the planned queries (see siSearchFill). */
+int siSearchFillPlanned(const siIndex* ix, void* plan, size_t nq, int order,
+                        const uint64_t* offsets, uint32_t* idx, void* stream);
+
+/* Payload fill pass for the planned queries (see siSearchValuesFill). */
+int siSearchValuesFillPlanned(const siIndex* ix, void* plan, size_t nq, const uint64_t* offsets,
+                              int32_t* values, void* stream);
+
+/* ---- Batched queries on device buffers ------------------------------------
+ * These plan the batch internally (stream-ordered scratch) and then run. */
 
 /* upper_bound for n values: out[k] = last position with starts[pos] <= xs[k], or
  * SI_INDEX_NONE.  Replaces superintervals.hpp:501-516 / c_superintervals.h:537-568. */
@@ -126,13 +154,14 @@ int siCount(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint32
 int siHasOverlaps(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint8_t* out,
                   void* stream);
 
-/* Workspace bytes siSearchOffsets needs for nq queries. */
+/* Workspace bytes siSearchOffsets needs for nq queries (either coordinate width). */
 size_t siSearchWorkspaceBytes(size_t nq);
 
 /* First pass of search_idxs: offsets[0..nq] (nq + 1 entries) = exclusive prefix sum of
  * the per-query overlap counts; offsets[nq] is the total.  workspace may be NULL (the
  * library then allocates stream-ordered memory) or point to siSearchWorkspaceBytes(nq)
- * device bytes.  counts may be NULL or receive the per-query counts. */
+ * device bytes; it then holds the batch's plan, usable with siSearchFillPlanned.
+ * counts may be NULL or receive the per-query counts. */
 int siSearchOffsets(const siIndex* ix, const void* qs, const void* qe, size_t nq,
                     uint32_t* counts, uint64_t* offsets, void* workspace, size_t workspace_bytes,
                     void* stream);

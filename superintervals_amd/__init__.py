@@ -245,6 +245,33 @@ class DeviceIndex:
     def workspace_bytes(nq: int) -> int:
         return int(N.lib.siSearchWorkspaceBytes(nq))
 
+    # ---- planned batches (torch tensors) ------------------------------------
+    def plan_bytes(self, nq: int) -> int:
+        return int(N.lib.siPlanBytes(self._h, nq))
+
+    def plan_device(self, qs_t, qe_t, plan_t, stream=None):
+        """Sort the batch by end and compute upper_bound positions into plan_t (a device
+        tensor of at least plan_bytes(nq) bytes)."""
+        N.check(N.lib.siPlanQueries(self._h, _ptr(qs_t), _ptr(qe_t), qs_t.numel(), _ptr(plan_t),
+                                    plan_t.numel() * plan_t.element_size(), _stream_of(stream)),
+                "siPlanQueries")
+
+    def count_planned(self, plan_t, nq, counts_t, stream=None):
+        N.check(N.lib.siCountPlanned(self._h, _ptr(plan_t), nq, _ptr(counts_t), _stream_of(stream)),
+                "siCountPlanned")
+
+    def offsets_planned(self, plan_t, nq, offsets_t, counts_t=None, stream=None):
+        N.check(N.lib.siSearchOffsetsPlanned(self._h, _ptr(plan_t), nq, _ptr(counts_t), _ptr(offsets_t),
+                                             _stream_of(stream)), "siSearchOffsetsPlanned")
+
+    def fill_planned(self, plan_t, nq, offsets_t, idx_t, order="desc", stream=None):
+        N.check(N.lib.siSearchFillPlanned(self._h, _ptr(plan_t), nq, _ORDERS[order], _ptr(offsets_t),
+                                          _ptr(idx_t), _stream_of(stream)), "siSearchFillPlanned")
+
+    def values_fill_planned(self, plan_t, nq, offsets_t, vals_t, stream=None):
+        N.check(N.lib.siSearchValuesFillPlanned(self._h, _ptr(plan_t), nq, _ptr(offsets_t), _ptr(vals_t),
+                                                _stream_of(stream)), "siSearchValuesFillPlanned")
+
 
 class IntervalMap:
     """The reference's Python IntervalMap (intervalmap.pyx:17-819), query path only.

@@ -55,6 +55,12 @@ _SIGS = {
     "siSearchValuesFill": (_i, [_vp, _vp, _vp, _sz, _vp, _vp, _vp]),
     "siCoverage": (_i, [_vp, _vp, _vp, _sz, _vp, _vp, _vp]),
     "siWalkStats": (_i, [_vp, _vp, _vp, _sz, _vp, _vp]),
+    "siPlanBytes": (_sz, [_vp, _sz]),
+    "siPlanQueries": (_i, [_vp, _vp, _vp, _sz, _vp, _sz, _vp]),
+    "siCountPlanned": (_i, [_vp, _vp, _sz, _vp, _vp]),
+    "siSearchOffsetsPlanned": (_i, [_vp, _vp, _sz, _vp, _vp, _vp]),
+    "siSearchFillPlanned": (_i, [_vp, _vp, _sz, _i, _vp, _vp, _vp]),
+    "siSearchValuesFillPlanned": (_i, [_vp, _vp, _sz, _vp, _vp, _vp]),
     "siCountHost": (_i, [_vp, _vp, _vp, _sz, _vp]),
     "siSearchHost": (_i, [_vp, _vp, _vp, _sz, _i, _vp, _vp, _vp, _sz, _vp]),
     "siUpperBoundHost": (_i, [_vp, _vp, _sz, _vp]),

@@ -1,0 +1,70 @@
+// si_internal.h -- the device index object shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "si_common.h"
+#include "si_kernels.h"
+
+using namespace sik;
+
+// ============================================================================
+// siIndex
+// ============================================================================
+struct siIndex {
+    int cb = 4;
+    size_t n = 0;
+    bool built = false;
+    int device = 0;
+    void* d_starts = nullptr;
+    void* d_ends = nullptr;
+    uint32_t* d_branch = nullptr;
+    uint32_t* d_perm = nullptr;
+    int32_t* d_values = nullptr;
+    void* d_tree = nullptr;   // levels 1.. of the max tree, contiguous
+    int nlev = 0;
+    size_t lvl_off[MAX_LEVELS] = {0};   // element offsets into d_tree (level >= 1)
+    uint32_t lvl_size[MAX_LEVELS] = {0};
+    double last_build_ms = 0.0;
+
+    void release() {
+        hipFree(d_starts); hipFree(d_ends); hipFree(d_branch); hipFree(d_perm);
+        hipFree(d_values); hipFree(d_tree);
+        d_starts = d_ends = d_tree = nullptr;
+        d_branch = d_perm = nullptr;
+        d_values = nullptr;
+        n = 0;
+        nlev = 0;
+        built = false;
+    }
+
+    template <typename S>
+    IndexView<S> view() const {
+        IndexView<S> v;
+        v.starts = (const S*)d_starts;
+        v.ends = (const S*)d_ends;
+        v.branch = d_branch;
+        v.n = (uint32_t)n;
+        return v;
+    }
+
+    template <typename S>
+    TreeDesc<S> tree() const {
+        TreeDesc<S> t;
+        t.nlev = nlev;
+        t.lvl[0] = (const S*)d_ends;
+        t.size[0] = (uint32_t)n;
+        for (int l = 1; l < nlev; ++l) {
+            t.lvl[l] = (const S*)d_tree + lvl_off[l];
+            t.size[l] = lvl_size[l];
+        }
+        for (int l = nlev; l < MAX_LEVELS; ++l) { t.lvl[l] = nullptr; t.size[l] = 0; }
+        return t;
+    }
+};
+
+inline int check_ix(const siIndex* ix) {
+    if (!ix) return si_set_error(SI_ERR_ARG, "null index");
+    if (!ix->built) return si_set_error(SI_ERR_NOT_BUILT, "index has not been built");
+    return SI_OK;
+}

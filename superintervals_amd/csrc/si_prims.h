@@ -1,0 +1,374 @@
+// si_prims.h -- device primitives shared by the build and query translation units:
+// wave/block scans and reductions, a three-kernel exclusive scan and a stable LSD
+// radix sort of 64-bit keys with a 32-bit payload.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "si_common.h"
+#include "si_kernels.h"
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess)                                                             \
+            return si_set_error(SI_ERR_HIP, "%s failed: %s (%s:%d)", #expr,               \
+                                hipGetErrorString(_e), __FILE__, __LINE__);               \
+    } while (0)
+
+#define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
+
+namespace sik {
+namespace {
+
+inline unsigned grid_for(size_t n, unsigned per_block) {
+    return (unsigned)((n + per_block - 1) / per_block);
+}
+
+// ============================================================================
+// Wave / block primitives
+// ============================================================================
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        T t = __shfl_up(v, d, WAVE);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        T t = __shfl_xor(v, d, WAVE);
+        v = t > v ? t : v;
+    }
+    return v;
+}
+
+// Exclusive scan across a 256-thread block; *total receives the block sum.
+template <typename T>
+__device__ __forceinline__ T block_excl_scan256(T v, T* s_w /*[4]*/, T* total) {
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    T inc = wave_incl_scan(v);
+    if (lane == WAVE - 1) s_w[w] = inc;
+    __syncthreads();
+    T pre = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        T x = s_w[k];
+        pre += (k < w) ? x : (T)0;
+        tot += x;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + inc - v;
+}
+
+// ============================================================================
+// Exclusive scan (reduce -> scan of tile sums -> downsweep), 4096 items per tile
+// ============================================================================
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const TI* __restrict__ in, size_t n,
+                                                              TO* __restrict__ tile_sums) {
+    __shared__ TO s_w[4];
+    size_t base = (size_t)blockIdx.x * SCAN_TILE;
+    TO sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        size_t i = base + (size_t)k * SCAN_THREADS + threadIdx.x;
+        if (i < n) sum += (TO)in[i];
+    }
+    TO tot;
+    block_excl_scan256<TO>(sum, s_w, &tot);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+// Single block: exclusive scan of nt tile sums in place; *grand receives the total.
+template <typename TO>
+__global__ __launch_bounds__(1024) void k_scan_tiles(TO* __restrict__ ts, size_t nt, TO* grand) {
+    __shared__ TO s_part[1024];
+    __shared__ TO s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (size_t base = 0; base < nt; base += 1024) {
+        size_t i = base + threadIdx.x;
+        TO v = (i < nt) ? ts[i] : (TO)0;
+        // inclusive scan over 1024 threads: wave scan then wave totals
+        TO inc = wave_incl_scan(v);
+        const int lane = lane_id(), w = threadIdx.x >> 6;
+        if (lane == WAVE - 1) s_part[w] = inc;
+        __syncthreads();
+        if (threadIdx.x < 16) {
+            TO x = s_part[threadIdx.x];
+            TO acc = 0;
+            for (int k = 0; k < 16; ++k) {
+                TO y = s_part[k];
+                acc += (k < (int)threadIdx.x) ? y : (TO)0;
+            }
+            s_part[16 + threadIdx.x] = acc;
+            (void)x;
+        }
+        __syncthreads();
+        TO excl = s_carry + s_part[16 + w] + inc - v;
+        __syncthreads();
+        if (i < nt) ts[i] = excl;
+        if (threadIdx.x == 1023) s_carry = excl + v;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && grand) *grand = s_carry;
+}
+
+// Downsweep: out[i] = tile prefix + exclusive scan within the tile.  Each thread owns
+// SCAN_ITEMS consecutive items.  in and out may alias.
+template <typename TI, typename TO>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const TI* in, size_t n,
+                                                            const TO* __restrict__ tile_pre,
+                                                            TO* out) {
+    __shared__ TO s_w[4];
+    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    TO v[SCAN_ITEMS];
+    TO sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        size_t i = base + k;
+        v[k] = (i < n) ? (TO)in[i] : (TO)0;
+        sum += v[k];
+    }
+    TO tot;
+    TO run = block_excl_scan256<TO>(sum, s_w, &tot) + tile_pre[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        size_t i = base + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+}
+
+// Exclusive scan of n items (TI) into out (TO); out may alias in when TI == TO.
+// If total != nullptr (device pointer) the grand total is stored there.
+// ws must hold at least scan_ws_items(n) TO values.
+inline size_t scan_ws_items(size_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+
+template <typename TI, typename TO>
+int device_excl_scan(const TI* in, size_t n, TO* out, TO* total, TO* ws, hipStream_t st) {
+    if (n == 0) {
+        if (total) HIP_TRY(hipMemsetAsync(total, 0, sizeof(TO), st));
+        return SI_OK;
+    }
+    size_t nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+    k_scan_reduce<TI, TO><<<(unsigned)nt, SCAN_THREADS, 0, st>>>(in, n, ws);
+    LAUNCH_CHECK();
+    k_scan_tiles<TO><<<1, 1024, 0, st>>>(ws, nt, total);
+    LAUNCH_CHECK();
+    k_scan_down<TI, TO><<<(unsigned)nt, SCAN_THREADS, 0, st>>>(in, n, ws, out);
+    LAUNCH_CHECK();
+    return SI_OK;
+}
+
+// ============================================================================
+// Radix sort: stable LSD over 64-bit keys with a 32-bit payload
+// ============================================================================
+constexpr int RS_THREADS = 256;
+constexpr int RS_ROUNDS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;   // 4096 items per tile
+constexpr int RS_BINS = 256;
+
+// Histograms of all eight digits at once (to skip constant-digit passes).
+__global__ __launch_bounds__(256) void k_rs_hist8(const uint64_t* __restrict__ keys, size_t n,
+                                                   uint32_t* __restrict__ hist8 /*[8][256]*/) {
+    __shared__ uint32_t h[8][RS_BINS];
+    for (int k = threadIdx.x; k < 8 * RS_BINS; k += blockDim.x) (&h[0][0])[k] = 0;
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        uint64_t key = keys[i];
+#pragma unroll
+        for (int d = 0; d < 8; ++d) atomicAdd(&h[d][(key >> (8 * d)) & 255], 1u);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 8 * RS_BINS; k += blockDim.x) {
+        uint32_t c = (&h[0][0])[k];
+        if (c) atomicAdd(&hist8[k], c);
+    }
+}
+
+// Per-tile digit counts, digit-major: th[d * ntiles + tile].
+__global__ __launch_bounds__(RS_THREADS) void k_rs_tile_hist(const uint64_t* __restrict__ keys,
+                                                             size_t n, int shift,
+                                                             uint32_t* __restrict__ th,
+                                                             uint32_t ntiles) {
+    __shared__ uint32_t h[RS_BINS];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    size_t base = (size_t)blockIdx.x * RS_TILE;
+#pragma unroll 4
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        size_t i = base + (size_t)r * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1u);
+    }
+    __syncthreads();
+    th[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter.  Each round ranks 256 items (4 waves x 64 lanes, in input order) by
+// matching digits with 8 ballots; ranks accumulate across rounds in LDS, the tile is
+// staged in LDS in digit order and written out so equal digits land contiguously.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
+    uint32_t* __restrict__ vout, size_t n, int shift, const uint32_t* __restrict__ th_counts,
+    const uint32_t* __restrict__ th_base, uint32_t ntiles) {
+    __shared__ uint64_t s_key[RS_TILE];
+    __shared__ uint32_t s_val[RS_TILE];
+    __shared__ uint32_t s_wcnt[4][RS_BINS];
+    __shared__ uint32_t s_run[RS_BINS];
+    __shared__ uint32_t s_dstart[RS_BINS];
+    __shared__ uint32_t s_w[4];
+
+    const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+    const uint32_t tile = blockIdx.x;
+    const size_t base = (size_t)tile * RS_TILE;
+    const uint32_t tile_n = (uint32_t)min((size_t)RS_TILE, n - base);
+
+    // digit starts within the tile = exclusive scan of this tile's digit counts
+    uint32_t my_cnt = th_counts[(size_t)tid * ntiles + tile];
+    uint32_t tot;
+    s_dstart[tid] = block_excl_scan256<uint32_t>(my_cnt, s_w, &tot);
+    s_run[tid] = 0;
+    __syncthreads();
+
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const uint32_t li = (uint32_t)r * RS_THREADS + tid;
+        const bool valid = li < tile_n;
+        uint64_t key = valid ? kin[base + li] : 0ull;
+        uint32_t val = valid ? vin[base + li] : 0u;
+        uint32_t digit = (uint32_t)(key >> shift) & 255u;
+        for (int k = lane; k < RS_BINS; k += WAVE) s_wcnt[w][k] = 0;
+        __syncthreads();
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            uint64_t bal = __ballot((digit >> b) & 1u);
+            peers &= ((digit >> b) & 1u) ? bal : ~bal;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+        if (valid && rank == 0) s_wcnt[w][digit] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {   // per digit: prefix over the four waves, advance the running count
+            uint32_t c0 = s_wcnt[0][tid], c1 = s_wcnt[1][tid], c2 = s_wcnt[2][tid], c3 = s_wcnt[3][tid];
+            uint32_t run = s_run[tid];
+            s_wcnt[0][tid] = run;
+            s_wcnt[1][tid] = run + c0;
+            s_wcnt[2][tid] = run + c0 + c1;
+            s_wcnt[3][tid] = run + c0 + c1 + c2;
+            s_run[tid] = run + c0 + c1 + c2 + c3;
+        }
+        __syncthreads();
+        if (valid) {
+            uint32_t slot = s_dstart[digit] + s_wcnt[w][digit] + rank;
+            s_key[slot] = key;
+            s_val[slot] = val;
+        }
+        __syncthreads();
+    }
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const uint32_t slot = (uint32_t)r * RS_THREADS + tid;
+        if (slot < tile_n) {
+            uint64_t key = s_key[slot];
+            uint32_t digit = (uint32_t)(key >> shift) & 255u;
+            size_t dst = (size_t)th_base[(size_t)digit * ntiles + tile] + (slot - s_dstart[digit]);
+            kout[dst] = key;
+            vout[dst] = s_val[slot];
+        }
+    }
+}
+
+// Temporary device memory of a radix sort of n items (see rs_temp_bytes).
+struct RsTemp {
+    uint32_t* hist8;   // [8][256] digit histograms
+    uint32_t* th;      // per-tile digit counts, digit-major
+    uint32_t* tb;      // their exclusive scan
+    uint32_t* ws;      // scan workspace
+};
+
+inline size_t rs_ntiles(size_t n) { return (n + RS_TILE - 1) / RS_TILE; }
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+inline size_t rs_temp_bytes(size_t n) {
+    const size_t nth = rs_ntiles(n) * RS_BINS;
+    return align256(8 * RS_BINS * 4) + 2 * align256(nth * 4) + align256(scan_ws_items(nth) * 4);
+}
+
+inline RsTemp rs_temp_carve(void* base, size_t n) {
+    const size_t nth = rs_ntiles(n) * RS_BINS;
+    char* p = (char*)base;
+    RsTemp t;
+    t.hist8 = (uint32_t*)p; p += align256(8 * RS_BINS * 4);
+    t.th = (uint32_t*)p; p += align256(nth * 4);
+    t.tb = (uint32_t*)p; p += align256(nth * 4);
+    t.ws = (uint32_t*)p;
+    return t;
+}
+
+__global__ void k_zero_u32(uint32_t* __restrict__ p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Stable sort of (keys, vals) by key digits [digit_lo, digit_hi) (8-bit digits,
+// least significant first).  Digits that are equal for every key are skipped, which
+// needs one host read-back of the digit histograms.  tmp comes from rs_temp_carve.  On
+// return *keys_sorted / *vals_sorted point at whichever buffer holds the result.
+int radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, size_t n,
+                     int digit_lo, int digit_hi, const RsTemp& tmp, uint64_t** keys_sorted,
+                     uint32_t** vals_sorted, hipStream_t st) {
+    *keys_sorted = k0;
+    *vals_sorted = v0;
+    if (n < 2) return SI_OK;
+    k_zero_u32<<<grid_for(8 * RS_BINS, 256), 256, 0, st>>>(tmp.hist8, 8 * RS_BINS);
+    LAUNCH_CHECK();
+    unsigned hb = std::min<unsigned>(grid_for(n, 256 * 16), 2048);
+    k_rs_hist8<<<hb, 256, 0, st>>>(k0, n, tmp.hist8);
+    LAUNCH_CHECK();
+    uint32_t h8[8 * RS_BINS];
+    HIP_TRY(hipMemcpyAsync(h8, tmp.hist8, sizeof(h8), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+
+    const uint32_t ntiles = (uint32_t)rs_ntiles(n);
+    const size_t nth = (size_t)ntiles * RS_BINS;
+    uint64_t *ka = k0, *kb = k1;
+    uint32_t *va = v0, *vb = v1;
+    for (int d = digit_lo; d < digit_hi; ++d) {
+        bool constant = false;
+        for (int b = 0; b < RS_BINS; ++b)
+            if (h8[d * RS_BINS + b] == n) { constant = true; break; }
+        if (constant) continue;
+        const int shift = 8 * d;
+        k_rs_tile_hist<<<ntiles, RS_THREADS, 0, st>>>(ka, n, shift, tmp.th, ntiles);
+        LAUNCH_CHECK();
+        int rc = device_excl_scan<uint32_t, uint32_t>(tmp.th, nth, tmp.tb, nullptr, tmp.ws, st);
+        if (rc != SI_OK) return rc;
+        k_rs_scatter<<<ntiles, RS_THREADS, 0, st>>>(ka, va, kb, vb, n, shift, tmp.th, tmp.tb, ntiles);
+        LAUNCH_CHECK();
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    *keys_sorted = ka;
+    *vals_sorted = va;
+    return SI_OK;
+}
+
+}  // namespace
+}  // namespace sik

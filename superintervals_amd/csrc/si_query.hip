@@ -1,0 +1,735 @@
+// si_query.hip -- batched overlap queries on the device index.
+//
+// Replaces superintervals.hpp:501-516 (upper_bound), :651-825 (count), :865-871
+// (has_overlaps), :879-905 (search_idxs), :551-579 (search_values) and
+// c_superintervals.h:537-756 (upperBound, anyOverlaps, searchValues, searchIdxs,
+// countOverlaps) with batch kernels.
+//
+// A batch runs in three stages.
+//  1. Plan: the queries are sorted by end (stable LSD radix sort, payload = query
+//     number) so the lanes of a wave walk neighbouring parts of the index and share
+//     cache lines; each sorted query gets its upper_bound position.
+//  2. Count: a persistent kernel walks every query.  One lane owns one query; when a
+//     lane's query finishes it takes the next one from a pool of 64 sorted queries the
+//     wave grabs from a global counter (work stealing), so lanes stay busy however
+//     uneven the hit counts are.  counts[q] / offsets are written in input order.
+//  3. Fill: the same kernel writes each query's positions at its CSR offset.
+//
+// The walk is the reference's backward superset walk taken one aligned 64-byte line of
+// ends at a time.  In the line [b0, b0 + W) holding the current position cur, every
+// j in [b0, cur] with ends[j] >= s is a hit (the hit set is {j <= upper_bound :
+// ends[j] >= s}).  The walk leaves the line to b0 - 1 when b0 itself is a hit.
+// Otherwise the reference walk would enter the run of misses [b0, x] at its top x
+// (just below the lowest hit, or cur when there is none) and follow branch[] from
+// there; that chain's first element below b0 is the nearest j < b0 with
+// ends[j] >= max(ends[b0..x]), which is branch[a] for the leftmost maximum a of
+// ends[b0..x].  One line load and at most one branch load replace the reference's
+// per-element steps while visiting the same hits in the same order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "si_internal.h"
+#include "si_prims.h"
+
+namespace {
+
+constexpr int Q_THREADS = 256;
+constexpr uint32_t POOL = 64;
+
+enum WalkMode { W_COUNT = 0, W_DESC = 1, W_VECTOR = 2, W_VALUES = 3 };
+
+// ============================================================================
+// Plan
+// ============================================================================
+template <typename S>
+struct PlanView {
+    uint32_t* counter;   // work-stealing counter of the walk kernels
+    uint32_t* q;         // input query number of sorted position k
+    S* s;                // query start of sorted position k
+    uint32_t* idx;       // upper_bound of sorted position k
+};
+
+// Byte layout of a plan of nq queries (coordinate width cb):
+//   [256 B counters][q: 4 nq][s: cb nq][idx: 4 nq][scratch: keys 2 x 8 nq, vals 2 x 4 nq,
+//   radix temporaries]; the scratch is reused as the offsets scan workspace.
+struct PlanLayout {
+    size_t off_q, off_s, off_idx, off_scratch, off_k0, off_k1, off_v0, off_v1, off_rs, total;
+    PlanLayout(int cb, size_t nq) {
+        off_q = 256;
+        off_s = off_q + align256(4 * nq);
+        off_idx = off_s + align256((size_t)cb * nq);
+        off_scratch = off_idx + align256(4 * nq);
+        off_k0 = off_scratch;
+        off_k1 = off_k0 + align256(8 * nq);
+        off_v0 = off_k1 + align256(8 * nq);
+        off_v1 = off_v0 + align256(4 * nq);
+        off_rs = off_v1 + align256(4 * nq);
+        size_t scan = align256(scan_ws_items(nq) * 8);
+        total = std::max(off_rs + rs_temp_bytes(nq), off_scratch + scan);
+    }
+};
+
+template <typename S>
+PlanView<S> plan_view(void* base, size_t nq) {
+    PlanLayout L(sizeof(S), nq);
+    char* p = (char*)base;
+    PlanView<S> v;
+    v.counter = (uint32_t*)p;
+    v.q = (uint32_t*)(p + L.off_q);
+    v.s = (S*)(p + L.off_s);
+    v.idx = (uint32_t*)(p + L.off_idx);
+    return v;
+}
+
+// int32: key = (biased end << 32) | query number, payload = start bits.  Only the high
+// four digits are sorted, so the key's low half carries the query number.
+__global__ void k_plan_keys_i32(const int32_t* __restrict__ qs, const int32_t* __restrict__ qe,
+                                size_t nq, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nq) {
+        key[k] = ((uint64_t)((uint32_t)qe[k] ^ 0x80000000u) << 32) | (uint32_t)k;
+        val[k] = (uint32_t)qs[k];
+    }
+}
+
+// int64: key = biased end, payload = query number.
+__global__ void k_plan_keys_i64(const int64_t* __restrict__ qe, size_t nq, uint64_t* __restrict__ key,
+                                uint32_t* __restrict__ val) {
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nq) {
+        key[k] = (uint64_t)qe[k] ^ 0x8000000000000000ull;
+        val[k] = (uint32_t)k;
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(Q_THREADS) void k_plan_finish(IndexView<S> ix, const uint64_t* __restrict__ key,
+                                                           const uint32_t* __restrict__ val,
+                                                           const S* __restrict__ qs, size_t nq,
+                                                           PlanView<S> pv) {
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) pv.counter[0] = 0;
+    if (k >= nq) return;
+    uint64_t kk = key[k];
+    uint32_t q;
+    S s, e;
+    if (sizeof(S) == 4) {
+        q = (uint32_t)kk;
+        e = (S)(int32_t)((uint32_t)(kk >> 32) ^ 0x80000000u);
+        s = (S)(int32_t)val[k];
+    } else {
+        q = val[k];
+        e = (S)(int64_t)(kk ^ 0x8000000000000000ull);
+        s = qs[q];
+    }
+    pv.q[k] = q;
+    pv.s[k] = s;
+    pv.idx[k] = upper_bound_pos(ix.starts, ix.n, e);
+}
+
+template <typename S>
+int plan_impl(const siIndex* ix, const S* qs, const S* qe, size_t nq, void* plan, hipStream_t st) {
+    PlanLayout L(sizeof(S), nq);
+    char* p = (char*)plan;
+    uint64_t* k0 = (uint64_t*)(p + L.off_k0);
+    uint64_t* k1 = (uint64_t*)(p + L.off_k1);
+    uint32_t* v0 = (uint32_t*)(p + L.off_v0);
+    uint32_t* v1 = (uint32_t*)(p + L.off_v1);
+    RsTemp rt = rs_temp_carve(p + L.off_rs, nq);
+    const unsigned g = grid_for(nq, Q_THREADS);
+    uint64_t* ks;
+    uint32_t* vs;
+    int rc;
+    if (sizeof(S) == 4) {
+        k_plan_keys_i32<<<g, Q_THREADS, 0, st>>>((const int32_t*)qs, (const int32_t*)qe, nq, k0, v0);
+        LAUNCH_CHECK();
+        rc = radix_sort_pairs(k0, v0, k1, v1, nq, 4, 8, rt, &ks, &vs, st);
+    } else {
+        k_plan_keys_i64<<<g, Q_THREADS, 0, st>>>((const int64_t*)qe, nq, k0, v0);
+        LAUNCH_CHECK();
+        rc = radix_sort_pairs(k0, v0, k1, v1, nq, 0, 8, rt, &ks, &vs, st);
+    }
+    if (rc != SI_OK) return rc;
+    k_plan_finish<S><<<g, Q_THREADS, 0, st>>>(ix->view<S>(), ks, vs, qs, nq, plan_view<S>(plan, nq));
+    LAUNCH_CHECK();
+    return SI_OK;
+}
+
+// ============================================================================
+// The block walk
+// ============================================================================
+template <typename S>
+struct Line {
+    static constexpr int W = 64 / sizeof(S);
+    S e[W];
+};
+
+template <typename S>
+__device__ __forceinline__ void load_line(const S* __restrict__ ends, uint32_t b0, Line<S>& L) {
+    const uint4* p = reinterpret_cast<const uint4*>(ends + b0);
+    uint4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = p[k];
+    memcpy(L.e, v, 64);
+}
+
+// Bit k set when position b0 + k (k <= kc) is a hit.
+template <typename S>
+__device__ __forceinline__ uint32_t hit_mask(const Line<S>& L, uint32_t kc, S s) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < Line<S>::W; ++k) m |= (((uint32_t)k <= kc) & (L.e[k] >= s) ? 1u : 0u) << k;
+    return m;
+}
+
+// Next position after the line, given its hit mask (see the file comment).
+template <typename S>
+__device__ __forceinline__ uint32_t leave_line(const IndexView<S>& ix, const Line<S>& L, uint32_t b0,
+                                               uint32_t kc, uint32_t m) {
+    if (m & 1u) return b0 - 1u;   // b0 == 0 wraps to NONE
+    const uint32_t x = m ? (uint32_t)(__builtin_ctz(m) - 1) : kc;
+    S best = L.e[0];
+    uint32_t a = 0;
+#pragma unroll
+    for (int k = 1; k < Line<S>::W; ++k) {
+        bool take = ((uint32_t)k <= x) & (L.e[k] > best);
+        best = take ? L.e[k] : best;
+        a = take ? (uint32_t)k : a;
+    }
+    return ix.branch[b0 + a];
+}
+
+template <typename S, int MODE>
+__global__ __launch_bounds__(Q_THREADS) void k_walk(IndexView<S> ix, PlanView<S> pv, uint32_t nq,
+                                                    uint32_t* __restrict__ counts,
+                                                    uint64_t* __restrict__ counts64,
+                                                    const uint64_t* __restrict__ offsets,
+                                                    uint32_t* __restrict__ out_idx,
+                                                    const int32_t* __restrict__ vals,
+                                                    int32_t* __restrict__ out_val) {
+    constexpr int W = Line<S>::W;
+    const int lane = lane_id();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t pool_base = 0, pool_left = 0;
+    bool exhausted = false;
+    bool has = false;
+    bool lead = false;
+    uint32_t q = 0, cur = NONE, cnt = 0, top = 0;
+    S s = 0;
+    uint64_t o = 0;
+    while (true) {
+        // ---- refill idle lanes from the wave's pool of sorted queries
+        const bool need = !has;
+        const uint64_t nm = __ballot(need);
+        if (nm) {
+            if (pool_left == 0 && !exhausted) {
+                uint32_t b = 0;
+                if (lane == 0) b = atomicAdd(pv.counter, POOL);
+                b = __shfl(b, 0, WAVE);
+                if (b >= nq) exhausted = true;
+                else { pool_base = b; pool_left = min(POOL, nq - b); }
+            }
+            const uint32_t r = (uint32_t)__popcll(nm & lt);
+            if (need && r < pool_left) {
+                const uint32_t k = pool_base + r;
+                q = pv.q[k];
+                s = pv.s[k];
+                cur = pv.idx[k];
+                cnt = 0;
+                has = true;
+                if (MODE != W_COUNT) o = offsets[q];
+                if (MODE == W_VECTOR) { top = cur; lead = true; }
+            }
+            const uint32_t take = min((uint32_t)__popcll(nm), pool_left);
+            pool_base += take;
+            pool_left -= take;
+        }
+        if (!__any(has)) {
+            if (exhausted) break;
+            continue;
+        }
+        if (has && cur != NONE) {
+            const uint32_t b0 = cur & ~(uint32_t)(W - 1);
+            const uint32_t kc = cur - b0;
+            Line<S> L;
+            load_line(ix.ends, b0, L);
+            const uint32_t m = hit_mask(L, kc, s);
+            if (MODE == W_VECTOR && lead) {
+                // leading contiguous run of hits, written ascending once its bottom is known
+                const uint32_t inv = ~m & ((2u << kc) - 1u);
+                uint32_t lo;
+                if (inv == 0) {
+                    cur = b0 - 1u;
+                    lo = 0;
+                } else {
+                    const uint32_t hm = 31u - (uint32_t)__builtin_clz(inv);
+                    cur = b0 + hm;   // the first miss below the run; the walk resumes there
+                    lo = cur + 1u;
+                }
+                if (inv != 0 || cur == NONE) {
+                    for (uint32_t j = lo; j <= top; ++j) out_idx[o++] = j;
+                    lead = false;
+                }
+            } else {
+                if (MODE == W_COUNT) {
+                    cnt += (uint32_t)__popc(m);
+                } else {
+#pragma unroll
+                    for (int k = W - 1; k >= 0; --k) {
+                        if ((m >> k) & 1u) {
+                            if (MODE == W_VALUES) out_val[o++] = vals[b0 + k];
+                            else out_idx[o++] = b0 + (uint32_t)k;
+                        }
+                    }
+                }
+                cur = leave_line(ix, L, b0, kc, m);
+            }
+        }
+        if (has && cur == NONE) {
+            if (MODE == W_COUNT) {
+                if (counts) counts[q] = cnt;
+                if (counts64) counts64[q] = cnt;
+            } else if (MODE == W_VECTOR && lead) {
+                for (uint32_t j = 0; j <= top && top != NONE; ++j) out_idx[o++] = j;
+                lead = false;
+            }
+            has = false;
+        }
+    }
+}
+
+// ============================================================================
+// Per-lane kernels over unsorted queries (single-purpose queries)
+// ============================================================================
+template <typename S>
+__global__ __launch_bounds__(Q_THREADS) void k_upper_bound(IndexView<S> ix, const S* __restrict__ xs,
+                                                           size_t n, uint32_t* __restrict__ out) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) out[q] = upper_bound_pos(ix.starts, ix.n, xs[q]);
+}
+
+template <typename S>
+__global__ __launch_bounds__(Q_THREADS) void k_has_overlaps(IndexView<S> ix, const S* __restrict__ qs,
+                                                            const S* __restrict__ qe, size_t nq,
+                                                            uint8_t* __restrict__ out) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    uint32_t i = upper_bound_pos(ix.starts, ix.n, qe[q]);
+    out[q] = (i != NONE && qs[q] <= ix.ends[i]) ? 1 : 0;
+}
+
+// Reference walk, element by element, counting its reads (c_superintervals.h:610-643).
+template <typename S>
+__global__ __launch_bounds__(Q_THREADS) void k_walk_stats(IndexView<S> ix, const S* __restrict__ qs,
+                                                          const S* __restrict__ qe, size_t nq,
+                                                          unsigned long long* __restrict__ totals) {
+    __shared__ unsigned long long s_t[4];
+    if (threadIdx.x < 4) s_t[threadIdx.x] = 0;
+    __syncthreads();
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long h = 0, v = 0, j = 0, f = 0;
+    if (q < nq) {
+        const S s = qs[q];
+        uint32_t i = upper_bound_pos(ix.starts, ix.n, qe[q]);
+        f = (i != NONE) ? 1 : 0;
+        while (i != NONE) {
+            ++v;
+            if (s <= ix.ends[i]) { ++h; --i; }
+            else { i = ix.branch[i]; ++j; }
+        }
+    }
+    h = wave_incl_scan(h); v = wave_incl_scan(v); j = wave_incl_scan(j); f = wave_incl_scan(f);
+    if (lane_id() == WAVE - 1) {
+        atomicAdd(&s_t[0], h); atomicAdd(&s_t[1], v); atomicAdd(&s_t[2], j); atomicAdd(&s_t[3], f);
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 && s_t[threadIdx.x]) atomicAdd(&totals[threadIdx.x], s_t[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(Q_THREADS) void k_coverage_i32(IndexView<int32_t> ix,
+                                                            const int32_t* __restrict__ qs,
+                                                            const int32_t* __restrict__ qe, size_t nq,
+                                                            uint32_t* __restrict__ cnt,
+                                                            int32_t* __restrict__ cov) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int32_t s = qs[q], e = qe[q];
+    uint32_t i = upper_bound_pos(ix.starts, ix.n, e);
+    uint32_t c = 0, acc = 0;   // unsigned accumulation = int32 wraparound
+    while (i != NONE) {
+        int32_t en = ix.ends[i];
+        if (s <= en) {
+            int32_t hi = en < e ? en : e;
+            int32_t st = ix.starts[i];
+            int32_t lo = st > s ? st : s;
+            ++c;
+            acc += (uint32_t)hi - (uint32_t)lo;
+            --i;
+        } else {
+            i = ix.branch[i];
+        }
+    }
+    if (cnt) cnt[q] = c;
+    if (cov) cov[q] = (int32_t)acc;
+}
+
+// ============================================================================
+// Launch helpers
+// ============================================================================
+#define DISPATCH_S(ix, ...)                                                  \
+    ((ix)->cb == 4 ? [&] { using S = int32_t; return __VA_ARGS__; }()        \
+                   : [&] { using S = int64_t; return __VA_ARGS__; }())
+
+// Resident waves for the persistent walk kernel, capped by the work available.
+template <typename S, int MODE>
+unsigned walk_grid(size_t nq) {
+    static unsigned blocks_full = 0;
+    if (blocks_full == 0) {
+        int dev = 0, cus = 0, per = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_walk<S, MODE>, Q_THREADS, 0);
+        blocks_full = (unsigned)std::max(1, cus * std::max(per, 1));
+    }
+    const unsigned need = (unsigned)((nq + (size_t)Q_THREADS - 1) / Q_THREADS);
+    return std::max(1u, std::min(blocks_full, need));
+}
+
+template <typename S, int MODE>
+int launch_walk(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint64_t* counts64,
+                const uint64_t* offsets, uint32_t* out_idx, int32_t* out_val, hipStream_t st) {
+    if (nq == 0) return SI_OK;
+    PlanView<S> pv = plan_view<S>(plan, nq);
+    k_zero_u32<<<1, 64, 0, st>>>(pv.counter, 1);
+    LAUNCH_CHECK();
+    k_walk<S, MODE><<<walk_grid<S, MODE>(nq), Q_THREADS, 0, st>>>(
+        ix->view<S>(), pv, (uint32_t)nq, counts, counts64, offsets, out_idx, ix->d_values, out_val);
+    LAUNCH_CHECK();
+    return SI_OK;
+}
+
+int check_plan(const siIndex* ix, const void* plan, size_t plan_bytes, size_t nq) {
+    if (!plan) return si_set_error(SI_ERR_ARG, "null plan");
+    if (nq >= (size_t)0x7FFFFFFF) return si_set_error(SI_ERR_ARG, "batch of %zu queries exceeds 2^31-1", nq);
+    const size_t need = PlanLayout(ix->cb, nq).total;
+    if (plan_bytes < need) return si_set_error(SI_ERR_ARG, "plan buffer too small: %zu < %zu bytes", plan_bytes, need);
+    return SI_OK;
+}
+
+// Stream-ordered scratch plan for the unplanned entry points.
+struct ScopedPlan {
+    void* p = nullptr;
+    hipStream_t st;
+    explicit ScopedPlan(hipStream_t s) : st(s) {}
+    int alloc(size_t bytes) {
+        hipError_t e = hipMallocAsync(&p, bytes, st);
+        if (e != hipSuccess) { p = nullptr; return si_set_error(SI_ERR_NOMEM, "hipMallocAsync(%zu): %s", bytes, hipGetErrorString(e)); }
+        return SI_OK;
+    }
+    ~ScopedPlan() { if (p) (void)hipFreeAsync(p, st); }
+};
+
+}  // namespace
+
+// ============================================================================
+// Planned batch entry points
+// ============================================================================
+SI_EXPORT size_t siPlanBytes(const siIndex* ix, size_t nq) {
+    return PlanLayout(ix ? ix->cb : 8, nq).total;
+}
+
+SI_EXPORT int siPlanQueries(const siIndex* ix, const void* qs, const void* qe, size_t nq, void* plan,
+                            size_t plan_bytes, void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if ((rc = check_plan(ix, plan, plan_bytes, nq))) return rc;
+    if (nq == 0) return SI_OK;
+    if (!qs || !qe) return si_set_error(SI_ERR_ARG, "null queries");
+    hipStream_t st = (hipStream_t)stream;
+    return DISPATCH_S(ix, plan_impl<S>(ix, (const S*)qs, (const S*)qe, nq, plan, st));
+}
+
+SI_EXPORT int siCountPlanned(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (!plan) return si_set_error(SI_ERR_ARG, "null plan");
+    if (nq && !counts) return si_set_error(SI_ERR_ARG, "null counts");
+    hipStream_t st = (hipStream_t)stream;
+    return DISPATCH_S(ix, (launch_walk<S, W_COUNT>(ix, plan, nq, counts, nullptr, nullptr, nullptr, nullptr, st)));
+}
+
+SI_EXPORT int siSearchOffsetsPlanned(const siIndex* ix, void* plan, size_t nq, uint32_t* counts,
+                                     uint64_t* offsets, void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (!plan || !offsets) return si_set_error(SI_ERR_ARG, "null plan/offsets");
+    hipStream_t st = (hipStream_t)stream;
+    rc = DISPATCH_S(ix, (launch_walk<S, W_COUNT>(ix, plan, nq, counts, offsets, nullptr, nullptr, nullptr, st)));
+    if (rc) return rc;
+    uint64_t* ws = (uint64_t*)((char*)plan + PlanLayout(ix->cb, nq).off_scratch);
+    return device_excl_scan<uint64_t, uint64_t>(offsets, nq, offsets, offsets + nq, ws, st);
+}
+
+SI_EXPORT int siSearchFillPlanned(const siIndex* ix, void* plan, size_t nq, int order,
+                                  const uint64_t* offsets, uint32_t* idx, void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (order != SI_ORDER_DESC && order != SI_ORDER_VECTOR) return si_set_error(SI_ERR_ARG, "bad order %d", order);
+    if (nq && (!plan || !offsets || !idx)) return si_set_error(SI_ERR_ARG, "null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    if (order == SI_ORDER_DESC)
+        return DISPATCH_S(ix, (launch_walk<S, W_DESC>(ix, plan, nq, nullptr, nullptr, offsets, idx, nullptr, st)));
+    return DISPATCH_S(ix, (launch_walk<S, W_VECTOR>(ix, plan, nq, nullptr, nullptr, offsets, idx, nullptr, st)));
+}
+
+SI_EXPORT int siSearchValuesFillPlanned(const siIndex* ix, void* plan, size_t nq, const uint64_t* offsets,
+                                        int32_t* values, void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (nq && (!plan || !offsets || !values)) return si_set_error(SI_ERR_ARG, "null buffer");
+    if (!ix->d_values) return si_set_error(SI_ERR_ARG, "index was built without values");
+    hipStream_t st = (hipStream_t)stream;
+    return DISPATCH_S(ix, (launch_walk<S, W_VALUES>(ix, plan, nq, nullptr, nullptr, offsets, nullptr, values, st)));
+}
+
+// ============================================================================
+// Unplanned batch entry points (plan into stream-ordered scratch, then run)
+// ============================================================================
+SI_EXPORT int siCount(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint32_t* counts,
+                      void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (nq == 0) return SI_OK;
+    if (!qs || !qe || !counts) return si_set_error(SI_ERR_ARG, "null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    ScopedPlan P(st);
+    const size_t pb = siPlanBytes(ix, nq);
+    if ((rc = P.alloc(pb))) return rc;
+    if ((rc = siPlanQueries(ix, qs, qe, nq, P.p, pb, stream))) return rc;
+    return siCountPlanned(ix, P.p, nq, counts, stream);
+}
+
+SI_EXPORT size_t siSearchWorkspaceBytes(size_t nq) { return PlanLayout(8, nq).total; }
+
+SI_EXPORT int siSearchOffsets(const siIndex* ix, const void* qs, const void* qe, size_t nq,
+                              uint32_t* counts, uint64_t* offsets, void* workspace,
+                              size_t workspace_bytes, void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (!offsets || (nq && (!qs || !qe))) return si_set_error(SI_ERR_ARG, "null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    ScopedPlan P(st);
+    void* plan = workspace;
+    size_t pb = workspace_bytes;
+    if (!plan) {
+        pb = siPlanBytes(ix, nq);
+        if ((rc = P.alloc(pb))) return rc;
+        plan = P.p;
+    }
+    if ((rc = siPlanQueries(ix, qs, qe, nq, plan, pb, stream))) return rc;
+    if (nq == 0) {
+        HIP_TRY(hipMemsetAsync(offsets, 0, sizeof(uint64_t), st));
+        return SI_OK;
+    }
+    return siSearchOffsetsPlanned(ix, plan, nq, counts, offsets, stream);
+}
+
+SI_EXPORT int siSearchFill(const siIndex* ix, const void* qs, const void* qe, size_t nq, int order,
+                           const uint64_t* offsets, uint32_t* idx, void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (order != SI_ORDER_DESC && order != SI_ORDER_VECTOR) return si_set_error(SI_ERR_ARG, "bad order %d", order);
+    if (nq == 0) return SI_OK;
+    if (!qs || !qe || !offsets || !idx) return si_set_error(SI_ERR_ARG, "null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    ScopedPlan P(st);
+    const size_t pb = siPlanBytes(ix, nq);
+    if ((rc = P.alloc(pb))) return rc;
+    if ((rc = siPlanQueries(ix, qs, qe, nq, P.p, pb, stream))) return rc;
+    return siSearchFillPlanned(ix, P.p, nq, order, offsets, idx, stream);
+}
+
+SI_EXPORT int siSearchValuesFill(const siIndex* ix, const void* qs, const void* qe, size_t nq,
+                                 const uint64_t* offsets, int32_t* values, void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (nq == 0) return SI_OK;
+    if (!qs || !qe || !offsets || !values) return si_set_error(SI_ERR_ARG, "null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    ScopedPlan P(st);
+    const size_t pb = siPlanBytes(ix, nq);
+    if ((rc = P.alloc(pb))) return rc;
+    if ((rc = siPlanQueries(ix, qs, qe, nq, P.p, pb, stream))) return rc;
+    return siSearchValuesFillPlanned(ix, P.p, nq, offsets, values, stream);
+}
+
+SI_EXPORT int siUpperBound(const siIndex* ix, const void* xs, size_t n, uint32_t* out, void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (n == 0) return SI_OK;
+    if (!xs || !out) return si_set_error(SI_ERR_ARG, "null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    return DISPATCH_S(ix, ([&]() -> int {
+        k_upper_bound<S><<<grid_for(n, Q_THREADS), Q_THREADS, 0, st>>>(ix->view<S>(), (const S*)xs, n, out);
+        LAUNCH_CHECK();
+        return SI_OK;
+    }()));
+}
+
+SI_EXPORT int siHasOverlaps(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint8_t* out,
+                            void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (nq == 0) return SI_OK;
+    if (!qs || !qe || !out) return si_set_error(SI_ERR_ARG, "null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    return DISPATCH_S(ix, ([&]() -> int {
+        k_has_overlaps<S><<<grid_for(nq, Q_THREADS), Q_THREADS, 0, st>>>(ix->view<S>(), (const S*)qs,
+                                                                        (const S*)qe, nq, out);
+        LAUNCH_CHECK();
+        return SI_OK;
+    }()));
+}
+
+SI_EXPORT int siCoverage(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint32_t* count_out,
+                         int32_t* cov_out, void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (ix->cb != 4) return si_set_error(SI_ERR_UNSUPPORTED, "coverage is defined for int32 coordinates");
+    if (nq == 0) return SI_OK;
+    if (!qs || !qe) return si_set_error(SI_ERR_ARG, "null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    k_coverage_i32<<<grid_for(nq, Q_THREADS), Q_THREADS, 0, st>>>(ix->view<int32_t>(), (const int32_t*)qs,
+                                                                 (const int32_t*)qe, nq, count_out, cov_out);
+    LAUNCH_CHECK();
+    return SI_OK;
+}
+
+SI_EXPORT int siWalkStats(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint64_t* totals,
+                          void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (!totals) return si_set_error(SI_ERR_ARG, "null totals");
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipMemsetAsync(totals, 0, 4 * sizeof(uint64_t), st));
+    if (nq == 0) return SI_OK;
+    return DISPATCH_S(ix, ([&]() -> int {
+        k_walk_stats<S><<<grid_for(nq, Q_THREADS), Q_THREADS, 0, st>>>(ix->view<S>(), (const S*)qs, (const S*)qe,
+                                                                      nq, (unsigned long long*)totals);
+        LAUNCH_CHECK();
+        return SI_OK;
+    }()));
+}
+
+// ============================================================================
+// Host-buffer forms
+// ============================================================================
+namespace {
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    int alloc(size_t bytes) {
+        if (bytes == 0) bytes = 8;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) { p = nullptr; return si_set_error(SI_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e)); }
+        return SI_OK;
+    }
+};
+
+int upload_queries(const siIndex* ix, const void* qs, const void* qe, size_t nq, DevBuf& dq) {
+    const size_t qb = nq * ix->cb;
+    int rc = dq.alloc(2 * qb);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(dq.p, qs, qb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy((char*)dq.p + qb, qe, qb, hipMemcpyHostToDevice));
+    return SI_OK;
+}
+}  // namespace
+
+SI_EXPORT int siCountHost(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint32_t* counts) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (nq == 0) return SI_OK;
+    if (!qs || !qe || !counts) return si_set_error(SI_ERR_ARG, "null buffer");
+    DevBuf dq, dc;
+    if ((rc = upload_queries(ix, qs, qe, nq, dq)) || (rc = dc.alloc(nq * 4))) return rc;
+    rc = siCount(ix, dq.p, (char*)dq.p + nq * ix->cb, nq, (uint32_t*)dc.p, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(counts, dc.p, nq * 4, hipMemcpyDeviceToHost));
+    return SI_OK;
+}
+
+SI_EXPORT int siUpperBoundHost(const siIndex* ix, const void* xs, size_t n, uint32_t* out) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (n == 0) return SI_OK;
+    DevBuf dx, dout;
+    if ((rc = dx.alloc(n * ix->cb)) || (rc = dout.alloc(n * 4))) return rc;
+    HIP_TRY(hipMemcpy(dx.p, xs, n * ix->cb, hipMemcpyHostToDevice));
+    rc = siUpperBound(ix, dx.p, n, (uint32_t*)dout.p, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(out, dout.p, n * 4, hipMemcpyDeviceToHost));
+    return SI_OK;
+}
+
+SI_EXPORT int siHasOverlapsHost(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint8_t* out) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (nq == 0) return SI_OK;
+    if (!qs || !qe || !out) return si_set_error(SI_ERR_ARG, "null buffer");
+    DevBuf dq, dout;
+    if ((rc = upload_queries(ix, qs, qe, nq, dq)) || (rc = dout.alloc(nq))) return rc;
+    rc = siHasOverlaps(ix, dq.p, (char*)dq.p + nq * ix->cb, nq, (uint8_t*)dout.p, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(out, dout.p, nq, hipMemcpyDeviceToHost));
+    return SI_OK;
+}
+
+SI_EXPORT int siCoverageHost(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint32_t* count_out,
+                             int32_t* cov_out) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (nq == 0) return SI_OK;
+    if (!qs || !qe || !count_out || !cov_out) return si_set_error(SI_ERR_ARG, "null buffer");
+    DevBuf dq, dc, dv;
+    if ((rc = upload_queries(ix, qs, qe, nq, dq)) || (rc = dc.alloc(nq * 4)) || (rc = dv.alloc(nq * 4))) return rc;
+    rc = siCoverage(ix, dq.p, (char*)dq.p + nq * ix->cb, nq, (uint32_t*)dc.p, (int32_t*)dv.p, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(count_out, dc.p, nq * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cov_out, dv.p, nq * 4, hipMemcpyDeviceToHost));
+    return SI_OK;
+}
+
+SI_EXPORT int siSearchHost(const siIndex* ix, const void* qs, const void* qe, size_t nq, int order,
+                           uint64_t* offsets, uint32_t* idx, int32_t* values, size_t cap, uint64_t* total) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (!offsets) return si_set_error(SI_ERR_ARG, "null offsets");
+    if (order != SI_ORDER_DESC && order != SI_ORDER_VECTOR) return si_set_error(SI_ERR_ARG, "bad order %d", order);
+    if (values && order != SI_ORDER_DESC) return si_set_error(SI_ERR_ARG, "values are emitted in descending order only");
+    if (total) *total = 0;
+    if (nq == 0) { offsets[0] = 0; return SI_OK; }
+    if (!qs || !qe) return si_set_error(SI_ERR_ARG, "null queries");
+    const size_t pb = siPlanBytes(ix, nq);
+    DevBuf dq, doff, dplan;
+    if ((rc = upload_queries(ix, qs, qe, nq, dq)) || (rc = doff.alloc((nq + 1) * 8)) || (rc = dplan.alloc(pb)))
+        return rc;
+    if ((rc = siPlanQueries(ix, dq.p, (char*)dq.p + nq * ix->cb, nq, dplan.p, pb, nullptr))) return rc;
+    if ((rc = siSearchOffsetsPlanned(ix, dplan.p, nq, nullptr, (uint64_t*)doff.p, nullptr))) return rc;
+    HIP_TRY(hipMemcpy(offsets, doff.p, (nq + 1) * 8, hipMemcpyDeviceToHost));
+    const uint64_t tot = offsets[nq];
+    if (total) *total = tot;
+    if (tot > cap) return si_set_error(SI_ERR_CAPACITY, "%llu results exceed capacity %zu", (unsigned long long)tot, cap);
+    if (tot == 0) return SI_OK;
+    if (!values && !idx) return si_set_error(SI_ERR_ARG, "null idx");
+    DevBuf dout;
+    if ((rc = dout.alloc(tot * 4))) return rc;
+    if (values) rc = siSearchValuesFillPlanned(ix, dplan.p, nq, (const uint64_t*)doff.p, (int32_t*)dout.p, nullptr);
+    else rc = siSearchFillPlanned(ix, dplan.p, nq, order, (const uint64_t*)doff.p, (uint32_t*)dout.p, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(values ? (void*)values : (void*)idx, dout.p, tot * 4, hipMemcpyDeviceToHost));
+    return SI_OK;
+}

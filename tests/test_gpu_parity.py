@@ -122,6 +122,42 @@ def test_random_vs_oracle(sia, orc, seed, dtype):
     ix.close()
 
 
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_planned_batch_reuse(sia, orc, dtype):
+    """One plan serves count, offsets and every fill mode, repeatedly, on one stream."""
+    import torch
+    print(f"[gpu] planned reuse dtype={np.dtype(dtype).name}", flush=True)
+    rec, s, e, qs, qe = load_golden("c1_shuffled")
+    s, e, qs, qe = (x.astype(dtype) for x in (s, e, qs, qe))
+    ix = _dev_index(sia, s, e, np.arange(s.size, dtype=np.int32))
+    oix = orc.build_index(s, e)
+    dev = torch.device("cuda:0")
+    tqs, tqe = torch.from_numpy(qs).to(dev), torch.from_numpy(qe).to(dev)
+    nq = qs.size
+    plan = torch.empty(ix.plan_bytes(nq), dtype=torch.uint8, device=dev)
+    ix.plan_device(tqs, tqe, plan)
+    counts = torch.empty(nq, dtype=torch.int32, device=dev)
+    off = torch.empty(nq + 1, dtype=torch.int64, device=dev)
+    for _ in range(2):
+        ix.count_planned(plan, nq, counts)
+        assert np.array_equal(counts.cpu().numpy().astype(np.uint32), rec["counts"])
+        ix.offsets_planned(plan, nq, off, counts)
+        assert np.array_equal(np.diff(off.cpu().numpy()), rec["counts"].astype(np.int64))
+    total = int(off[-1].item())
+    idx = torch.empty(total, dtype=torch.int32, device=dev)
+    offh = off.cpu().numpy().astype(np.uint64)
+    for order, key in (("desc", "hash_desc_digest"), ("vector", "hash_vec_digest"), ("desc", "hash_desc_digest")):
+        idx.fill_(-7)
+        ix.fill_planned(plan, nq, off, idx, order=order)
+        h = orc.hash_csr(offh, idx.cpu().numpy().view(np.uint32))
+        assert orc.digest(h) == int(rec[key]), order
+    vals = torch.empty(total, dtype=torch.int32, device=dev)
+    ix.values_fill_planned(plan, nq, off, vals)
+    ooff, oidx = orc.search_batch(oix, qs, qe, mode=0)
+    assert np.array_equal(vals.cpu().numpy(), oix["data"][oidx])
+    ix.close()
+
+
 def test_walk_stats_match_oracle(sia, orc):
     import torch
     rec, s, e, qs, qe = load_golden("c1_sorted")
