@@ -45,6 +45,7 @@ struct siIndex {
         v.ends = (const S*)d_ends;
         v.branch = d_branch;
         v.n = (uint32_t)n;
+        v.npad = (uint32_t)((n + 63) & ~(size_t)63);
         return v;
     }
 

@@ -1,8 +1,9 @@
 // si_kernels.h -- device kernels of the superset index (gfx950, wave64).
 //
 // Layout in HBM of a built index of n intervals (coordinate type S):
-//   starts[n], ends[n]   S, sorted by (start asc, end desc)
-//   branch[n]            uint32, nearest j < i with ends[j] >= ends[i], or NONE
+//   starts[n], ends[n]   S, sorted by (start asc, end desc), padded to a multiple of
+//                        64 entries (starts with the maximum, ends with the minimum)
+//   branch[n]            uint32, nearest j < i with ends[j] >= ends[i], or NONE (padded)
 //   perm[n]              uint32, input position of the interval at sorted position k
 //   values[n]            int32 payload in sorted order (optional)
 //   tree levels          S, a 64-ary max tree over ends: level l+1 holds the max of
@@ -32,6 +33,7 @@ struct IndexView {
     const S* __restrict__ ends;
     const uint32_t* __restrict__ branch;
     uint32_t n;
+    uint32_t npad;   // allocated length of starts/ends/branch (multiple of 64)
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }

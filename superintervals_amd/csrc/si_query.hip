@@ -9,10 +9,13 @@
 //  1. Plan: the queries are sorted by end (stable LSD radix sort, payload = query
 //     number) so the lanes of a wave walk neighbouring parts of the index and share
 //     cache lines; each sorted query gets its upper_bound position.
-//  2. Count: a persistent kernel walks every query.  One lane owns one query; when a
-//     lane's query finishes it takes the next one from a pool of 64 sorted queries the
-//     wave grabs from a global counter (work stealing), so lanes stay busy however
-//     uneven the hit counts are.  counts[q] / offsets are written in input order.
+//  2. Count: a persistent kernel walks every query.  A workgroup takes a tile of
+//     TILE consecutive sorted queries from a global counter, stages the window of
+//     ends[] and branch[] just below the tile's largest upper_bound in LDS (64 KB),
+//     and walks from LDS; the few reads below the window go to HBM.  One lane owns
+//     one query; when it finishes, the lane takes the next query from a pool of 64
+//     the wave grabs from the tile's LDS counter, so lanes stay busy however uneven
+//     the hit counts are.  counts[q] / offsets are written in input order.
 //  3. Fill: the same kernel writes each query's positions at its CSR offset.
 //
 // The walk is the reference's backward superset walk taken one aligned 64-byte line of
@@ -185,11 +188,11 @@ __device__ __forceinline__ uint32_t hit_mask(const Line<S>& L, uint32_t kc, S s)
     return m;
 }
 
-// Next position after the line, given its hit mask (see the file comment).
+// Offset in the line of the leftmost maximum of ends over the misses [b0, b0 + x], where
+// x is just below the lowest hit (or kc when the line has none).  Only called when b0
+// itself is not a hit; the walk then continues at branch[b0 + offset].
 template <typename S>
-__device__ __forceinline__ uint32_t leave_line(const IndexView<S>& ix, const Line<S>& L, uint32_t b0,
-                                               uint32_t kc, uint32_t m) {
-    if (m & 1u) return b0 - 1u;   // b0 == 0 wraps to NONE
+__device__ __forceinline__ uint32_t exit_pos(const Line<S>& L, uint32_t kc, uint32_t m) {
     const uint32_t x = m ? (uint32_t)(__builtin_ctz(m) - 1) : kc;
     S best = L.e[0];
     uint32_t a = 0;
@@ -199,11 +202,68 @@ __device__ __forceinline__ uint32_t leave_line(const IndexView<S>& ix, const Lin
         best = take ? L.e[k] : best;
         a = take ? (uint32_t)k : a;
     }
-    return ix.branch[b0 + a];
+    return a;
 }
 
+// Wave-cooperative output.  Every lane holds this step's segment (c values at out
+// offset o); the wave scans the counts and lane x writes element x of the concatenated
+// segments, so one store instruction covers 64 consecutive output slots of a few lists
+// instead of one slot in each of 64 lists.  All lanes execute the shuffles.
 template <typename S, int MODE>
-__global__ __launch_bounds__(Q_THREADS) void k_walk(IndexView<S> ix, PlanView<S> pv, uint32_t nq,
+__device__ __forceinline__ void emit_segments(uint32_t c, uint32_t m, uint32_t base, bool asc,
+                                              uint64_t o, int lane, uint32_t* __restrict__ out_idx,
+                                              const int32_t* __restrict__ vals,
+                                              int32_t* __restrict__ out_val) {
+    constexpr int W = Line<S>::W;
+    const uint32_t inc = wave_incl_scan(c);
+    const uint32_t total = __shfl(inc, WAVE - 1, WAVE);
+    if (total == 0) return;
+    const uint32_t pre = inc - c;
+    const uint32_t flag = asc ? 1u : 0u;
+    for (uint32_t x0 = 0; x0 < total; x0 += WAVE) {
+        const uint32_t x = x0 + (uint32_t)lane;
+        // owner: the largest lane l with pre_l <= x (it has c_l > 0 whenever x < total)
+        int l = 0;
+#pragma unroll
+        for (int stp = 32; stp >= 1; stp >>= 1) {
+            const uint32_t pc = __shfl(pre, l + stp, WAVE);
+            l = (pc <= x) ? l + stp : l;
+        }
+        const uint32_t r = x - __shfl(pre, l, WAVE);
+        const uint32_t om = __shfl(m, l, WAVE);
+        const uint32_t ob = __shfl(base, l, WAVE);
+        const uint32_t of = __shfl(flag, l, WAVE);
+        const uint64_t oo = __shfl(o, l, WAVE);
+        uint32_t pos;
+        if (of) {
+            pos = ob + r;
+        } else {
+            // the (r+1)-th highest set bit of om: largest p with popc(om >> p) >= r + 1
+            uint32_t p = 0;
+#pragma unroll
+            for (int sh = W / 2; sh >= 1; sh >>= 1)
+                p = ((uint32_t)__popc(om >> (p + sh)) >= r + 1u) ? p + sh : p;
+            pos = ob + p;
+        }
+        if (x < total) {
+            if (MODE == W_VALUES) out_val[oo + r] = vals[pos];
+            else out_idx[oo + r] = pos;
+        }
+    }
+}
+
+// LDS window of a tile: ends and branch of positions [wlo, wlo + WIN), which always
+// ends at or above the tile's largest upper_bound.  64 KB per workgroup.
+template <typename S> struct TileCfg;
+template <> struct TileCfg<int32_t> { static constexpr uint32_t WIN = 8128; };   // 8128 x 8 B
+template <> struct TileCfg<int64_t> { static constexpr uint32_t WIN = 5376; };   // 5376 x 12 B
+
+constexpr int T_THREADS = 512;   // 8 waves per workgroup, two workgroups per CU
+
+template <typename S, int MODE>
+__global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S> pv, uint32_t nq,
+                                                    uint32_t tile, uint32_t ntiles,
+                                                    uint32_t* __restrict__ tile_counter,
                                                     uint32_t* __restrict__ counts,
                                                     uint64_t* __restrict__ counts64,
                                                     const uint64_t* __restrict__ offsets,
@@ -211,93 +271,148 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
                                                     const int32_t* __restrict__ vals,
                                                     int32_t* __restrict__ out_val) {
     constexpr int W = Line<S>::W;
+    constexpr uint32_t WIN = TileCfg<S>::WIN;
+    __shared__ __align__(16) S s_end[WIN];
+    __shared__ __align__(16) uint32_t s_br[WIN];
+    __shared__ uint32_t s_tile, s_next;
     const int lane = lane_id();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    uint32_t pool_base = 0, pool_left = 0;
-    bool exhausted = false;
-    bool has = false;
-    bool lead = false;
-    uint32_t q = 0, cur = NONE, cnt = 0, top = 0;
-    S s = 0;
-    uint64_t o = 0;
+
     while (true) {
-        // ---- refill idle lanes from the wave's pool of sorted queries
-        const bool need = !has;
-        const uint64_t nm = __ballot(need);
-        if (nm) {
-            if (pool_left == 0 && !exhausted) {
-                uint32_t b = 0;
-                if (lane == 0) b = atomicAdd(pv.counter, POOL);
-                b = __shfl(b, 0, WAVE);
-                if (b >= nq) exhausted = true;
-                else { pool_base = b; pool_left = min(POOL, nq - b); }
-            }
-            const uint32_t r = (uint32_t)__popcll(nm & lt);
-            if (need && r < pool_left) {
-                const uint32_t k = pool_base + r;
-                q = pv.q[k];
-                s = pv.s[k];
-                cur = pv.idx[k];
-                cnt = 0;
-                has = true;
-                if (MODE != W_COUNT) o = offsets[q];
-                if (MODE == W_VECTOR) { top = cur; lead = true; }
-            }
-            const uint32_t take = min((uint32_t)__popcll(nm), pool_left);
-            pool_base += take;
-            pool_left -= take;
+        // ---- take the next tile (workgroup-level work stealing)
+        if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1u);
+        __syncthreads();
+        const uint32_t t = s_tile;
+        if (t >= ntiles) break;
+        const uint32_t t0 = t * tile, t1 = min(t0 + tile, nq);
+        // sorted by end, so the last query of the tile has the largest upper_bound
+        const uint32_t whi = pv.idx[t1 - 1];
+        uint32_t wlo = 0, wcnt = 0;
+        if (whi != NONE) {
+            const int64_t lo = (int64_t)(whi & ~15u) + 16 - (int64_t)WIN;
+            wlo = lo < 0 ? 0u : (uint32_t)lo;
+            wcnt = min(WIN, ix.npad - wlo);   // multiple of 16
         }
-        if (!__any(has)) {
-            if (exhausted) break;
-            continue;
+        {
+            const uint4* ge = reinterpret_cast<const uint4*>(ix.ends + wlo);
+            const uint4* gb = reinterpret_cast<const uint4*>(ix.branch + wlo);
+            uint4* le = reinterpret_cast<uint4*>(s_end);
+            uint4* lb = reinterpret_cast<uint4*>(s_br);
+            const uint32_t ne = wcnt * (uint32_t)sizeof(S) / 16u, nb = wcnt / 4u;
+            for (uint32_t j = threadIdx.x; j < ne; j += T_THREADS) le[j] = ge[j];
+            for (uint32_t j = threadIdx.x; j < nb; j += T_THREADS) lb[j] = gb[j];
         }
-        if (has && cur != NONE) {
-            const uint32_t b0 = cur & ~(uint32_t)(W - 1);
-            const uint32_t kc = cur - b0;
-            Line<S> L;
-            load_line(ix.ends, b0, L);
-            const uint32_t m = hit_mask(L, kc, s);
-            if (MODE == W_VECTOR && lead) {
-                // leading contiguous run of hits, written ascending once its bottom is known
-                const uint32_t inv = ~m & ((2u << kc) - 1u);
-                uint32_t lo;
-                if (inv == 0) {
-                    cur = b0 - 1u;
-                    lo = 0;
-                } else {
-                    const uint32_t hm = 31u - (uint32_t)__builtin_clz(inv);
-                    cur = b0 + hm;   // the first miss below the run; the walk resumes there
-                    lo = cur + 1u;
+        if (threadIdx.x == 0) s_next = t0;
+        __syncthreads();
+
+        // ---- walk the tile; each wave refills idle lanes from pools of sorted queries
+        uint32_t pool_base = 0, pool_left = 0;
+        bool exhausted = false, has = false, lead = false;
+        uint32_t q = 0, cur = NONE, cnt = 0, top = 0;
+        S s = 0;
+        uint64_t o = 0;
+        while (true) {
+            const bool need = !has;
+            const uint64_t nm = __ballot(need);
+            if (nm) {
+                if (pool_left == 0 && !exhausted) {
+                    uint32_t b = 0;
+                    if (lane == 0) b = atomicAdd(&s_next, POOL);
+                    b = __shfl(b, 0, WAVE);
+                    if (b >= t1) exhausted = true;
+                    else { pool_base = b; pool_left = min(POOL, t1 - b); }
                 }
-                if (inv != 0 || cur == NONE) {
-                    for (uint32_t j = lo; j <= top; ++j) out_idx[o++] = j;
-                    lead = false;
+                const uint32_t r = (uint32_t)__popcll(nm & lt);
+                if (need && r < pool_left) {
+                    const uint32_t k = pool_base + r;
+                    q = pv.q[k];
+                    s = pv.s[k];
+                    cur = pv.idx[k];
+                    cnt = 0;
+                    has = true;
+                    if (MODE != W_COUNT) o = offsets[q];
+                    if (MODE == W_VECTOR) { top = cur; lead = true; }
                 }
-            } else {
-                if (MODE == W_COUNT) {
-                    cnt += (uint32_t)__popc(m);
+                const uint32_t take = min((uint32_t)__popcll(nm), pool_left);
+                pool_base += take;
+                pool_left -= take;
+            }
+            if (!__any(has)) {
+                if (exhausted) break;
+                continue;
+            }
+            // this step's output segment of the lane: c_emit values at e_o, either the
+            // ascending range e_base .. e_base + c_emit - 1 or the set bits of e_m
+            // (descending) offset by e_base
+            uint32_t c_emit = 0, e_m = 0, e_base = 0;
+            bool e_asc = false;
+            uint64_t e_o = 0;
+            if (has && cur != NONE) {
+                const uint32_t b0 = cur & ~(uint32_t)(W - 1);
+                const uint32_t kc = cur - b0;
+                const bool in_win = b0 >= wlo;   // every position visited is <= whi
+                Line<S> L;
+                if (in_win) memcpy(L.e, &s_end[b0 - wlo], 64);
+                else load_line(ix.ends, b0, L);
+                const uint32_t m = hit_mask(L, kc, s);
+                if (MODE == W_VECTOR && lead) {
+                    // leading contiguous run of hits, written ascending once its bottom is known
+                    const uint32_t inv = ~m & ((2u << kc) - 1u);
+                    uint32_t lo;
+                    if (inv == 0) {
+                        cur = b0 - 1u;
+                        lo = 0;
+                    } else {
+                        const uint32_t hm = 31u - (uint32_t)__builtin_clz(inv);
+                        cur = b0 + hm;   // the first miss below the run; the walk resumes there
+                        lo = cur + 1u;
+                    }
+                    if (inv != 0 || cur == NONE) {
+                        c_emit = top - lo + 1u;
+                        e_base = lo;
+                        e_asc = true;
+                        e_o = o;
+                        o += c_emit;
+                        lead = false;
+                    }
                 } else {
-#pragma unroll
-                    for (int k = W - 1; k >= 0; --k) {
-                        if ((m >> k) & 1u) {
-                            if (MODE == W_VALUES) out_val[o++] = vals[b0 + k];
-                            else out_idx[o++] = b0 + (uint32_t)k;
-                        }
+                    if (MODE == W_COUNT) {
+                        cnt += (uint32_t)__popc(m);
+                    } else {
+                        c_emit = (uint32_t)__popc(m);
+                        e_m = m;
+                        e_base = b0;
+                        e_o = o;
+                        o += c_emit;
+                    }
+                    if (m & 1u) {
+                        cur = b0 - 1u;   // b0 == 0 wraps to NONE
+                    } else {
+                        const uint32_t a = exit_pos(L, kc, m);
+                        const uint32_t p = b0 + a;
+                        // LDS read always (clamped), global read only below the window; the
+                        // empty asm pins the LDS value so the two loads are not folded into
+                        // one generic (flat) load through a selected pointer
+                        uint32_t nb = s_br[in_win ? p - wlo : 0u];
+                        asm volatile("" : "+v"(nb));
+                        if (!in_win) nb = ix.branch[p];
+                        cur = nb;
                     }
                 }
-                cur = leave_line(ix, L, b0, kc, m);
             }
-        }
-        if (has && cur == NONE) {
-            if (MODE == W_COUNT) {
-                if (counts) counts[q] = cnt;
-                if (counts64) counts64[q] = cnt;
-            } else if (MODE == W_VECTOR && lead) {
-                for (uint32_t j = 0; j <= top && top != NONE; ++j) out_idx[o++] = j;
+            if (MODE != W_COUNT) {
+                emit_segments<S, MODE>(c_emit, e_m, e_base, e_asc, e_o, lane, out_idx, vals, out_val);
+            }
+            if (has && cur == NONE) {
+                if (MODE == W_COUNT) {
+                    if (counts) counts[q] = cnt;
+                    if (counts64) counts64[q] = cnt;
+                }
                 lead = false;
+                has = false;
             }
-            has = false;
         }
+        __syncthreads();   // LDS window and counters are reused by the next tile
     }
 }
 
@@ -383,19 +498,20 @@ __global__ __launch_bounds__(Q_THREADS) void k_coverage_i32(IndexView<int32_t> i
     ((ix)->cb == 4 ? [&] { using S = int32_t; return __VA_ARGS__; }()        \
                    : [&] { using S = int64_t; return __VA_ARGS__; }())
 
-// Resident waves for the persistent walk kernel, capped by the work available.
+// Queries per tile and resident workgroups of the persistent walk kernel.
+constexpr uint32_t TILE = 2048;
+
 template <typename S, int MODE>
-unsigned walk_grid(size_t nq) {
+unsigned walk_grid(uint32_t ntiles) {
     static unsigned blocks_full = 0;
     if (blocks_full == 0) {
         int dev = 0, cus = 0, per = 0;
         hipGetDevice(&dev);
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_walk<S, MODE>, Q_THREADS, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_walk<S, MODE>, T_THREADS, 0);
         blocks_full = (unsigned)std::max(1, cus * std::max(per, 1));
     }
-    const unsigned need = (unsigned)((nq + (size_t)Q_THREADS - 1) / Q_THREADS);
-    return std::max(1u, std::min(blocks_full, need));
+    return std::max(1u, std::min(blocks_full, ntiles));
 }
 
 template <typename S, int MODE>
@@ -403,10 +519,12 @@ int launch_walk(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint
                 const uint64_t* offsets, uint32_t* out_idx, int32_t* out_val, hipStream_t st) {
     if (nq == 0) return SI_OK;
     PlanView<S> pv = plan_view<S>(plan, nq);
+    const uint32_t ntiles = (uint32_t)((nq + TILE - 1) / TILE);
     k_zero_u32<<<1, 64, 0, st>>>(pv.counter, 1);
     LAUNCH_CHECK();
-    k_walk<S, MODE><<<walk_grid<S, MODE>(nq), Q_THREADS, 0, st>>>(
-        ix->view<S>(), pv, (uint32_t)nq, counts, counts64, offsets, out_idx, ix->d_values, out_val);
+    k_walk<S, MODE><<<walk_grid<S, MODE>(ntiles), T_THREADS, 0, st>>>(
+        ix->view<S>(), pv, (uint32_t)nq, TILE, ntiles, pv.counter, counts, counts64, offsets, out_idx,
+        ix->d_values, out_val);
     LAUNCH_CHECK();
     return SI_OK;
 }
