@@ -32,7 +32,18 @@ def parse():
     p.add_argument("--nq", type=int, default=10_000_000)
     p.add_argument("--cpu-sample", type=int, default=2_000_000)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="also launch one 1 GiB device copy (reference kernel for PMC byte counters)")
     return p.parse_args()
+
+
+def load_traffic():
+    """HBM bytes per launch of the walk kernels from the committed PMC summary, if any."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_workload2.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def cpu_baseline(s, e, qs, qe, sample):
@@ -79,6 +90,12 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if a.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.int32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     s, e = sia.workload_intervals(2, a.n, seed=1, sorted=True)
     qs, qe = sia.workload_queries(2, a.nq, seed=2)
     ix = sia.DeviceIndex(np.int32).build(s, e)
@@ -150,9 +167,14 @@ def main():
     }
     dom = max(("plan", t_plan), ("search_offsets", t_off), ("search_fill", t_fill), key=lambda x: x[1])[0]
     ach = b_count / t_count / 1e6
+    tr = load_traffic()
     roof = {"bound": "hbm", "kernel": "k_walk<int, count> (count pass, one launch over all queries)",
             "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+            "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": tr["count"]["bytes_per_launch"] if tr else None}
+    if tr:
+        kernels["count"]["hbm_bytes_pmc"] = tr["count"]["bytes_per_launch"]
+        kernels["search_fill"]["hbm_bytes_pmc"] = tr["fill"]["bytes_per_launch"]
 
     out = {
         "metric": "overlap queries/sec + achieved HBM GB/s (10M idx x 10M qry); p50/p99 batch latency",

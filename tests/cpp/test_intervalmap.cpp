@@ -1,0 +1,206 @@
+// C++ API test: si::IntervalMap<S, T> from include/superintervals.hpp on the GPU.
+//
+// Known answers are the ones the reference's own test suite asserts for the query path
+// (test/tests.cpp:63-238); the randomized part checks every query family against closed
+// forms computed here on the host from the sorted arrays.  Exit status 0 = pass.
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "superintervals.hpp"
+
+static int failures = 0;
+#define CHECK(cond)                                                                 \
+    do {                                                                            \
+        if (!(cond)) {                                                              \
+            std::fprintf(stderr, "FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond);    \
+            ++failures;                                                             \
+        }                                                                           \
+    } while (0)
+
+using Map = si::IntervalMap<int32_t, int>;
+
+static void known_answers() {
+    {   // tests.cpp:63-107
+        Map m;
+        CHECK(!m.has_overlaps(0, 1));
+        m.add(10, 20, 0); m.add(11, 12, -1); m.add(13, 14, -1); m.add(15, 16, -1); m.add(25, 29, 4);
+        m.build();
+        CHECK(m.has_overlaps(17, 30));
+        CHECK(!m.has_overlaps(1, 3));
+        std::vector<int> v;
+        m.search_values(17, 30, v);
+        CHECK(v.size() == 2 && v[0] == 4 && v[1] == 0);
+        CHECK(m.count(17, 30) == 2);
+        std::vector<size_t> ix;
+        m.search_idxs(17, 30, ix);
+        CHECK(ix.size() == 2 && ix[0] == 4 && ix[1] == 0);
+        std::vector<std::pair<int32_t, int32_t>> keys;
+        m.search_keys(17, 30, keys);
+        CHECK(keys.size() == 2 && keys[0].first == 25 && keys[1].second == 20);
+        std::pair<size_t, int32_t> cov{0, 0};
+        m.coverage(10, 29, cov);
+        CHECK(cov.second == 17);
+    }
+    {   // tests.cpp:113-133
+        Map m;
+        m.add(1, 10, 0);
+        m.build();
+        size_t n = 0;
+        int last = -1;
+        for (size_t i : m.search_idxs(5, 11)) { last = m.data[i]; ++n; }
+        CHECK(n == 1 && last == 0);
+        for (const auto& it : m.search_items(5, 11)) CHECK(it.start == 1 && it.end == 10 && it.data == 0);
+    }
+    {   // tests.cpp:146-156
+        Map m;
+        m.add(0, 250000, 0);
+        for (int s : {55, 115, 130, 281, 639, 842, 999, 1094, 1157, 1161, 1265, 1532, 1590, 1665, 1945, 2384, 2515})
+            m.add(s, s + 1000, -1);
+        m.build();
+        std::vector<int> a;
+        m.search_values(1377, 2377, a);
+        CHECK(a.size() == 12 && a.back() == 0);
+    }
+    {   // tests.cpp:161-171
+        Map m;
+        m.add(3, 40, 0); m.add(4, 5, 4); m.add(6, 7, 4); m.add(10, 31, 5); m.add(31, 32, 5);
+        m.build();
+        std::vector<int> a;
+        m.search_values(31, 32, a); CHECK(a.size() == 3); a.clear();
+        m.search_values(10, 11, a); CHECK(a.size() == 2); a.clear();
+        m.search_values(4, 7, a); CHECK(a.size() == 3);
+    }
+    {   // tests.cpp:175-186
+        Map m;
+        m.add(3, 40, 0); m.add(3, 40, 4); m.add(3, 40, 4); m.add(3, 4, 4); m.add(35, 50, 4);
+        m.add(40, 400, 5); m.add(40, 400, 4);
+        m.build();
+        std::vector<int> a;
+        m.search_values(38, 41, a); CHECK(a.size() == 6); a.clear();
+        m.search_values(41, 42, a); CHECK(a.size() == 3);
+    }
+    {   // tests.cpp:194-213
+        Map m;
+        m.add(1, 100, 0); m.add(30, 200, 7); m.add(40, 50, 6); m.add(60, 70, 7);
+        m.build();
+        std::vector<int> a;
+        m.search_values(55, 65, a);
+        CHECK(a.size() == 3);
+        std::pair<size_t, int32_t> cov{0, 0};
+        m.coverage(55, 65, cov);
+        CHECK(cov.first == 3 && cov.second == 25);
+    }
+    {   // tests.cpp:219-238
+        Map e;
+        e.build();
+        CHECK(e.count(1, 5) == 0 && !e.has_overlaps(1, 5));
+        Map one;
+        one.add(1, 10, 1);
+        one.build();
+        CHECK(one.count(1, 5) == 1 && one.count(11, 20) == 0);
+    }
+    {   // vector order: the leading run is inserted ascending (superintervals.hpp:891-895)
+        Map m;
+        int s[] = {1, 2, 4, 5, 6, 60}, e[] = {100, 3, 50, 50, 50, 70};
+        for (int k = 0; k < 6; ++k) m.add(s[k], e[k], 100 + k);
+        m.build();
+        std::vector<size_t> v;
+        m.search_idxs(10, 65, v);
+        CHECK((v == std::vector<size_t>{2, 3, 4, 5, 0}));
+        std::vector<size_t> it;
+        for (size_t i : m.search_idxs(10, 65)) it.push_back(i);
+        CHECK((it == std::vector<size_t>{5, 4, 3, 2, 0}));
+        std::vector<size_t> br(m.branch.begin(), m.branch.end());
+        CHECK((br == std::vector<size_t>{SIZE_MAX, 0, 0, 2, 3, 0}));
+        std::vector<int> vals;
+        m.search_values(10, 65, vals);
+        CHECK((vals == std::vector<int>{105, 104, 103, 102, 100}));
+    }
+}
+
+template <typename S>
+static void randomized(uint64_t seed) {
+    std::mt19937_64 rng(seed);
+    for (int round = 0; round < 4; ++round) {
+        const size_t n = 1 + rng() % 5000;
+        const int64_t span = 1000 + (int64_t)(rng() % 200000);
+        const S off = (sizeof(S) == 8) ? (S)((int64_t)1 << 36) : 0;
+        si::IntervalMap<S, int> m;
+        for (size_t i = 0; i < n; ++i) {
+            S s = off + (S)(rng() % span);
+            m.add(s, s + (S)(rng() % (1 + span / 50)), (int)i);
+        }
+        m.build();
+        // sorted by (start asc, end desc), branch = previous greater-or-equal end
+        for (size_t i = 1; i < n; ++i)
+            CHECK(m.starts[i - 1] < m.starts[i] || (m.starts[i - 1] == m.starts[i] && m.ends[i - 1] >= m.ends[i]));
+        for (size_t i = 0; i < n; i += 7) {
+            size_t want = SIZE_MAX;
+            for (size_t j = i; j-- > 0;) if (m.ends[j] >= m.ends[i]) { want = j; break; }
+            CHECK(m.branch[i] == want);
+        }
+        std::vector<S> qs(300), qe(300);
+        for (size_t q = 0; q < qs.size(); ++q) {
+            qs[q] = off + (S)(rng() % (span + 100)) - 50;
+            qe[q] = qs[q] + (S)(rng() % (span / 20 + 1));
+        }
+        std::vector<uint32_t> counts(qs.size());
+        CHECK(m.count_batch(qs.data(), qe.data(), qs.size(), counts.data()));
+        std::vector<uint64_t> off_v, off_d;
+        std::vector<uint32_t> idx_v, idx_d;
+        CHECK(m.search_idxs_batch(qs.data(), qe.data(), qs.size(), off_v, idx_v, true));
+        CHECK(m.search_idxs_batch(qs.data(), qe.data(), qs.size(), off_d, idx_d, false));
+        for (size_t q = 0; q < qs.size(); ++q) {
+            const S a = qs[q], b = qe[q];
+            size_t ub = (size_t)(std::upper_bound(m.starts.begin(), m.starts.end(), b) - m.starts.begin());
+            std::vector<size_t> strict;
+            for (size_t j = ub; j-- > 0;) if (m.ends[j] >= a) strict.push_back(j);
+            size_t i = ub;
+            while (i > 0 && m.ends[i - 1] >= a) --i;   // [i, ub) is the leading run
+            std::vector<size_t> vec;
+            for (size_t j = i; j < ub; ++j) vec.push_back(j);
+            for (size_t j : strict) if (j < i) vec.push_back(j);
+            CHECK(m.count(a, b) == strict.size());
+            CHECK(counts[q] == strict.size());
+            CHECK(m.upper_bound(b) == (ub == 0 ? SIZE_MAX : ub - 1));
+            std::vector<size_t> got;
+            m.search_idxs(a, b, got);
+            CHECK(got == vec);
+            std::vector<size_t> it;
+            for (size_t j : m.search_idxs(a, b)) it.push_back(j);
+            CHECK(it == strict);
+            CHECK(off_v[q + 1] - off_v[q] == vec.size() && off_d[q + 1] - off_d[q] == strict.size());
+            for (size_t k = 0; k < vec.size(); ++k) CHECK(idx_v[off_v[q] + k] == vec[k]);
+            for (size_t k = 0; k < strict.size(); ++k) CHECK(idx_d[off_d[q] + k] == strict[k]);
+            std::vector<int> vals;
+            m.search_values(a, b, vals);
+            CHECK(vals.size() == strict.size());
+            for (size_t k = 0; k < vals.size() && k < strict.size(); ++k) CHECK(vals[k] == m.data[strict[k]]);
+            CHECK(m.has_overlaps(a, b) == (ub > 0 && m.ends[ub - 1] >= a));
+        }
+        // payload follows its interval through the sort
+        for (size_t k = 0; k < n; ++k) CHECK(m.data[k] >= 0 && (size_t)m.data[k] < n);
+    }
+}
+
+int main() {
+    siIndex* probe = siIndexCreate(4);
+    if (!probe) {
+        std::fprintf(stderr, "no GPU: %s\n", siGetLastError());
+        return 2;
+    }
+    siIndexDestroy(probe);
+    known_answers();
+    randomized<int32_t>(1);
+    randomized<int64_t>(2);
+    if (failures) {
+        std::fprintf(stderr, "%d failure(s)\n", failures);
+        return 1;
+    }
+    std::printf("superintervals.hpp C++ API: all checks passed\n");
+    return 0;
+}

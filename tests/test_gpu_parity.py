@@ -314,6 +314,18 @@ def test_python_intervalmap_mirror(sia, orc):
         m2.count_batch([1, 2], [3])
 
 
+def test_cpp_api_program(sia):
+    """include/superintervals.hpp (si::IntervalMap<S, T>) through a compiled C++ program."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    exe = os.path.join(ROOT, "tests", "cpp", "test_intervalmap")
+    assert os.path.exists(exe), "build the C++ test first (__graft_entry__.build())"
+    print("[gpu] C++ API program", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+
+
 def test_full_size_workload2_properties(sia, orc):
     """10M intervals x 10M queries (workload 2): size-independent properties of the whole
     output plus an oracle check of a query sample."""
