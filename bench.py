@@ -32,6 +32,8 @@ def parse():
     p.add_argument("--nq", type=int, default=10_000_000)
     p.add_argument("--cpu-sample", type=int, default=2_000_000)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-tune", action="store_true", help="keep the library's default launch choices")
+    p.add_argument("--verbose", action="store_true")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also launch one 1 GiB device copy (reference kernel for PMC byte counters)")
     return p.parse_args()
@@ -115,6 +117,13 @@ def main():
     # reference-walk statistics for the algorithmic byte model
     hits, V, J, found = ix.walk_stats_device(tqs, tqe)
 
+    # launch choices: autotuned on this index and batch before the timed region
+    tuning = None
+    if not a.no_tune:
+        from superintervals_amd import autotune
+        tuning = autotune.load_or_tune(ix, plan, nq, off, idx, total, stream=stream, retune=True,
+                                       verbose=a.verbose)
+
     def step(evs=None):
         if evs:
             evs[0].record(stream)
@@ -152,6 +161,17 @@ def main():
     c1.record(stream)
     torch.cuda.synchronize()
     t_count = c0.elapsed_time(c1) / a.steps
+    # rank form of count (upper_bound from the plan + one search of the sorted ends)
+    from superintervals_amd import autotune
+    autotune.apply({"count_rank": 1})
+    ix.count_planned(plan, nq, counts, stream=stream)
+    c0.record(stream)
+    for _ in range(a.steps):
+        ix.count_planned(plan, nq, counts, stream=stream)
+    c1.record(stream)
+    torch.cuda.synchronize()
+    autotune.apply({"count_rank": 0})
+    t_count_rank = c0.elapsed_time(c1) / a.steps
 
     c = 4
     logn = math.ceil(math.log2(a.n))
@@ -159,6 +179,7 @@ def main():
     b_search = b_count + 4 * hits + 8 * nq
     kernels = {
         "count": {"ms": t_count, "alg_bytes": b_count, "GBps": b_count / t_count / 1e6},
+        "count_rank": {"ms": t_count_rank, "queries_per_s": nq / (t_count_rank / 1e3)},
         "plan": {"ms": t_plan},
         "search_offsets": {"ms": t_off},
         "search_fill": {"ms": t_fill},
@@ -186,7 +207,8 @@ def main():
         "dtype": "int32", "data": "synthetic (seeded log-normal generator, inputs resident in HBM)",
         "config": {"workload": "search_idxs two-pass, 10M int32 intervals x 10M queries (log-normal lengths)",
                    "intervals": a.n, "queries": nq, "hits": total, "order": "descending",
-                   "build_ms": round(build_ms, 2), "dominant_search_kernel": dom},
+                   "build_ms": round(build_ms, 2), "dominant_search_kernel": dom,
+                   "launch": tuning if tuning else "library defaults"},
         "roofline": roof,
         "kernels": kernels,
         "count_queries_per_s": nq / (t_count / 1e3),

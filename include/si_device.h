@@ -110,6 +110,18 @@ const uint32_t* siIndexDevicePerm(const siIndex* ix);
 /* Milliseconds spent in the last siIndexBuild on the device (sort, branch, trees). */
 double siIndexLastBuildMs(const siIndex* ix);
 
+/* ---- Launch tuning ----------------------------------------------------------
+ * Process-wide launch choices of the walk kernels (results never depend on them):
+ *   "tile"       sorted queries per workgroup tile (64 .. 2^20, default 2048)
+ *   "count_wgs"  resident workgroups per CU in the count pass (0 = occupancy limit)
+ *   "fill_wgs"   resident workgroups per CU in the fill passes (0 = occupancy limit)
+ *   "count_rank"   1: siCount* use the rank form (upper_bound + 1 - #{ends < s}, exact
+ *                  for s <= e; inverted queries walk) instead of the walk (default 0)
+ *   "offsets_rank" 1 (default): siSearchOffsets* count with the rank form; 0: the walk
+ * superintervals_amd/autotune.py measures candidates and stores the best per workload. */
+int siTuneSet(const char* name, int64_t value);
+int64_t siTuneGet(const char* name);   /* -1 for an unknown name */
+
 /* ---- Planned batches (device buffers) --------------------------------------
  * A plan holds one batch of queries sorted by end together with each query's
  * upper_bound position, so the count and fill kernels walk neighbouring queries in

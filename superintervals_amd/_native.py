@@ -55,6 +55,8 @@ _SIGS = {
     "siSearchValuesFill": (_i, [_vp, _vp, _vp, _sz, _vp, _vp, _vp]),
     "siCoverage": (_i, [_vp, _vp, _vp, _sz, _vp, _vp, _vp]),
     "siWalkStats": (_i, [_vp, _vp, _vp, _sz, _vp, _vp]),
+    "siTuneSet": (_i, [C.c_char_p, C.c_int64]),
+    "siTuneGet": (C.c_int64, [C.c_char_p]),
     "siPlanBytes": (_sz, [_vp, _sz]),
     "siPlanQueries": (_i, [_vp, _vp, _vp, _sz, _vp, _sz, _vp]),
     "siCountPlanned": (_i, [_vp, _vp, _sz, _vp, _vp]),

@@ -1,0 +1,124 @@
+"""Autotuner for the walk kernels' launch choices.
+
+The kernels' results never depend on these choices (tile size, resident workgroups per
+CU for the count and the fill pass); only their speed does.  tune() times every
+candidate on the caller's own index and planned batch with HIP events and keeps the
+fastest count + fill combination.  Results are cached per workload signature (coordinate
+width, log2 of the index size, log2 of the batch size, log2 of the mean hits per query)
+in a JSON file, so later runs of the same shape reuse them.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import time
+
+from . import _native as N
+
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "superintervals_amd", "tune.json")
+KNOBS = ("tile", "count_wgs", "fill_wgs", "count_rank", "offsets_rank")
+CANDIDATE_TILES = (512, 1024, 2048, 4096)
+CANDIDATE_WGS = (1, 0)     # 0 = the occupancy limit
+
+
+def current() -> dict:
+    return {k: int(N.lib.siTuneGet(k.encode())) for k in KNOBS}
+
+
+def apply(params: dict) -> None:
+    for k in KNOBS:
+        if k in params:
+            N.check(N.lib.siTuneSet(k.encode(), int(params[k])), f"siTuneSet({k})")
+
+
+def signature(coord_bytes: int, n: int, nq: int, hits: int) -> str:
+    mean = hits / max(nq, 1)
+    return "c%d_n%d_q%d_h%d" % (coord_bytes, int(math.log2(max(n, 1))), int(math.log2(max(nq, 1))),
+                                int(math.log2(max(mean, 1.0))))
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _save(path, table):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def tune(ix, plan_t, nq, offsets_t, idx_t, stream=None, reps=3, tiles=CANDIDATE_TILES,
+         wgs=CANDIDATE_WGS, verbose=False):
+    """Time count + fill for every candidate on a planned batch; apply and return the best.
+
+    ix is a DeviceIndex, plan_t a planned batch (plan_device), offsets_t the batch's
+    offsets (offsets_planned already run) and idx_t an output buffer of offsets[nq]
+    entries.  Returns (best params, table of measured ms)."""
+    import torch
+    stream = stream or torch.cuda.current_stream()
+    counts = torch.empty(nq, dtype=torch.int32, device=offsets_t.device)
+    saved = current()
+    results = []
+    for tile in tiles:
+        for cw in wgs:
+            for fw in wgs:
+                apply({"tile": tile, "count_wgs": cw, "fill_wgs": fw})
+                ix.count_planned(plan_t, nq, counts, stream=stream)       # warm
+                ix.fill_planned(plan_t, nq, offsets_t, idx_t, stream=stream)
+                e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+                e0.record(stream)
+                for _ in range(reps):
+                    ix.count_planned(plan_t, nq, counts, stream=stream)
+                e1.record(stream)
+                for _ in range(reps):
+                    ix.fill_planned(plan_t, nq, offsets_t, idx_t, stream=stream)
+                e2.record(stream)
+                e2.synchronize()
+                tc, tf = e0.elapsed_time(e1) / reps, e1.elapsed_time(e2) / reps
+                results.append({"tile": tile, "count_wgs": cw, "fill_wgs": fw, "count_ms": tc, "fill_ms": tf})
+                if verbose:
+                    print("tune tile=%d count_wgs=%d fill_wgs=%d: count %.3f ms fill %.3f ms" % (tile, cw, fw, tc, tf),
+                          flush=True)
+    apply(saved)
+    best_count = min(results, key=lambda r: r["count_ms"])
+    # count and fill share the tile; pick the tile by their sum, the wgs of each by its own time
+    by_tile = {}
+    for r in results:
+        c = by_tile.setdefault(r["tile"], {"count": r, "fill": r})
+        if r["count_ms"] < c["count"]["count_ms"]:
+            c["count"] = r
+        if r["fill_ms"] < c["fill"]["fill_ms"]:
+            c["fill"] = r
+    tile = min(by_tile, key=lambda t: by_tile[t]["count"]["count_ms"] + by_tile[t]["fill"]["fill_ms"])
+    best = {"tile": tile, "count_wgs": by_tile[tile]["count"]["count_wgs"],
+            "fill_wgs": by_tile[tile]["fill"]["fill_wgs"],
+            "count_ms": by_tile[tile]["count"]["count_ms"], "fill_ms": by_tile[tile]["fill"]["fill_ms"]}
+    del best_count
+    apply(best)
+    return best, results
+
+
+def load_or_tune(ix, plan_t, nq, offsets_t, idx_t, hits, cache=DEFAULT_CACHE, stream=None, retune=False,
+                 verbose=False):
+    """Apply cached choices for this workload signature, tuning (and caching) if absent."""
+    sig = signature(ix.dtype.itemsize, ix.n, nq, hits)
+    table = _load(cache)
+    if not retune and sig in table:
+        apply(table[sig])
+        return dict(table[sig], cached=True, signature=sig)
+    t0 = time.perf_counter()
+    best, _ = tune(ix, plan_t, nq, offsets_t, idx_t, stream=stream, verbose=verbose)
+    best = dict(best, tuned_s=round(time.perf_counter() - t0, 2))
+    table[sig] = best
+    try:
+        _save(cache, table)
+    except OSError:
+        pass
+    return dict(best, cached=False, signature=sig)

@@ -62,6 +62,27 @@ __global__ void k_pad(S* __restrict__ st, S* __restrict__ en, uint32_t* __restri
     }
 }
 
+// Order-preserving unsigned keys of ends (and back), for the sorted copy of ends.
+template <typename S>
+__global__ void k_end_keys(const S* __restrict__ e, uint64_t* __restrict__ key, uint32_t* __restrict__ val,
+                           size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        key[i] = (sizeof(S) == 4) ? (uint64_t)((uint32_t)e[i] ^ 0x80000000u)
+                                  : ((uint64_t)e[i] ^ 0x8000000000000000ull);
+        val[i] = (uint32_t)i;
+    }
+}
+
+template <typename S>
+__global__ void k_end_unkey(const uint64_t* __restrict__ key, S* __restrict__ out, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        out[i] = (sizeof(S) == 4) ? (S)(int32_t)((uint32_t)key[i] ^ 0x80000000u)
+                                  : (S)(int64_t)(key[i] ^ 0x8000000000000000ull);
+    }
+}
+
 // int32 composite key: (start asc, end desc) in one 64-bit unsigned order.
 __global__ void k_key_i32(const int32_t* __restrict__ s, const int32_t* __restrict__ e,
                           uint64_t* __restrict__ key, size_t n) {
@@ -275,6 +296,34 @@ int build_impl(siIndex* ix, const void* starts, const void* ends, const int32_t*
     ix->n = n;
     k_branch<S><<<g, 256, 0, st>>>(ix->tree<S>(), ix->d_branch);
     LAUNCH_CHECK();
+    // ends in ascending order, for the rank form of count
+    {
+        const size_t kb = align256(n * 8), vb = align256(n * 4);
+        char* tmp = nullptr;
+        HIP_TRY(hipMalloc(&tmp, 2 * kb + 2 * vb + rs_temp_bytes(n)));
+        uint64_t* k0 = (uint64_t*)tmp;
+        uint64_t* k1 = (uint64_t*)(tmp + kb);
+        uint32_t* v0 = (uint32_t*)(tmp + 2 * kb);
+        uint32_t* v1 = (uint32_t*)(tmp + 2 * kb + vb);
+        RsTemp rt = rs_temp_carve(tmp + 2 * kb + 2 * vb, n);
+        k_end_keys<S><<<g, 256, 0, st>>>((const S*)ix->d_ends, k0, v0, n);
+        LAUNCH_CHECK();
+        uint64_t* ks;
+        uint32_t* vs;
+        int rc = radix_sort_pairs(k0, v0, k1, v1, n, 0, (int)sizeof(S), rt, &ks, &vs, st);
+        if (rc == SI_OK) {
+            hipError_t e = hipMalloc(&ix->d_ends_sorted, n * sizeof(S));
+            if (e == hipSuccess) {
+                k_end_unkey<S><<<g, 256, 0, st>>>(ks, (S*)ix->d_ends_sorted, n);
+                e = hipGetLastError();
+            }
+            if (e != hipSuccess) rc = si_set_error(SI_ERR_HIP, "sorted ends: %s", hipGetErrorString(e));
+        }
+        hipError_t es = hipStreamSynchronize(st);
+        hipFree(tmp);
+        if (rc != SI_OK) return rc;
+        if (es != hipSuccess) return si_set_error(SI_ERR_HIP, "build: %s", hipGetErrorString(es));
+    }
     HIP_TRY(hipStreamSynchronize(st));
     if (!on_device) { hipFree(in_s); hipFree(in_e); if (in_v) hipFree(in_v); }
     ix->built = true;

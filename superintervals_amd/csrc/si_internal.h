@@ -22,6 +22,7 @@ struct siIndex {
     uint32_t* d_perm = nullptr;
     int32_t* d_values = nullptr;
     void* d_tree = nullptr;   // levels 1.. of the max tree, contiguous
+    void* d_ends_sorted = nullptr;   // ends in ascending order (rank count)
     int nlev = 0;
     size_t lvl_off[MAX_LEVELS] = {0};   // element offsets into d_tree (level >= 1)
     uint32_t lvl_size[MAX_LEVELS] = {0};
@@ -29,8 +30,8 @@ struct siIndex {
 
     void release() {
         hipFree(d_starts); hipFree(d_ends); hipFree(d_branch); hipFree(d_perm);
-        hipFree(d_values); hipFree(d_tree);
-        d_starts = d_ends = d_tree = nullptr;
+        hipFree(d_values); hipFree(d_tree); hipFree(d_ends_sorted);
+        d_starts = d_ends = d_tree = d_ends_sorted = nullptr;
         d_branch = d_perm = nullptr;
         d_values = nullptr;
         n = 0;

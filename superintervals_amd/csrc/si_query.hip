@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "si_internal.h"
@@ -52,18 +53,20 @@ struct PlanView {
     uint32_t* counter;   // work-stealing counter of the walk kernels
     uint32_t* q;         // input query number of sorted position k
     S* s;                // query start of sorted position k
+    S* e;                // query end of sorted position k
     uint32_t* idx;       // upper_bound of sorted position k
 };
 
 // Byte layout of a plan of nq queries (coordinate width cb):
-//   [256 B counters][q: 4 nq][s: cb nq][idx: 4 nq][scratch: keys 2 x 8 nq, vals 2 x 4 nq,
-//   radix temporaries]; the scratch is reused as the offsets scan workspace.
+//   [256 B counters][q: 4 nq][s: cb nq][e: cb nq][idx: 4 nq][scratch: keys 2 x 8 nq,
+//   vals 2 x 4 nq, radix temporaries]; the scratch is reused as the offsets scan workspace.
 struct PlanLayout {
-    size_t off_q, off_s, off_idx, off_scratch, off_k0, off_k1, off_v0, off_v1, off_rs, total;
+    size_t off_q, off_s, off_e, off_idx, off_scratch, off_k0, off_k1, off_v0, off_v1, off_rs, total;
     PlanLayout(int cb, size_t nq) {
         off_q = 256;
         off_s = off_q + align256(4 * nq);
-        off_idx = off_s + align256((size_t)cb * nq);
+        off_e = off_s + align256((size_t)cb * nq);
+        off_idx = off_e + align256((size_t)cb * nq);
         off_scratch = off_idx + align256(4 * nq);
         off_k0 = off_scratch;
         off_k1 = off_k0 + align256(8 * nq);
@@ -83,6 +86,7 @@ PlanView<S> plan_view(void* base, size_t nq) {
     v.counter = (uint32_t*)p;
     v.q = (uint32_t*)(p + L.off_q);
     v.s = (S*)(p + L.off_s);
+    v.e = (S*)(p + L.off_e);
     v.idx = (uint32_t*)(p + L.off_idx);
     return v;
 }
@@ -130,7 +134,44 @@ __global__ __launch_bounds__(Q_THREADS) void k_plan_finish(IndexView<S> ix, cons
     }
     pv.q[k] = q;
     pv.s[k] = s;
+    pv.e[k] = e;
     pv.idx[k] = upper_bound_pos(ix.starts, ix.n, e);
+}
+
+// Number of sorted ends below x (branchless lower_bound).
+template <typename S>
+__device__ __forceinline__ uint32_t count_below(const S* __restrict__ v, uint32_t n, S x) {
+    if (n == 0) return 0;
+    const S* base = v;
+    uint32_t len = n;
+    while (len > 1) {
+        uint32_t half = len >> 1;
+        base = (base[half] < x) ? base + half : base;
+        len -= half;
+    }
+    return (uint32_t)(base - v) + ((*base < x) ? 1u : 0u);
+}
+
+// Rank form of count for the planned queries: for s <= e every interval with end < s
+// starts before e, so #{starts <= e && ends >= s} = (upper_bound + 1) - #{ends < s}.
+// Inverted queries (s > e) take the reference walk.
+template <typename S>
+__global__ __launch_bounds__(Q_THREADS) void k_count_rank(IndexView<S> ix, const S* __restrict__ ends_sorted,
+                                                          PlanView<S> pv, size_t nq,
+                                                          uint32_t* __restrict__ counts,
+                                                          uint64_t* __restrict__ counts64) {
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nq) return;
+    const uint32_t i = pv.idx[k];
+    const S s = pv.s[k], e = pv.e[k];
+    uint32_t c = 0;
+    if (i != NONE) {
+        if (s <= e) c = i + 1u - count_below(ends_sorted, ix.n, s);
+        else c = walk_count(ix, i, s);
+    }
+    const uint32_t q = pv.q[k];
+    if (counts) counts[q] = c;
+    if (counts64) counts64[q] = c;
 }
 
 template <typename S>
@@ -310,7 +351,7 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
         bool exhausted = false, has = false, lead = false;
         uint32_t q = 0, cur = NONE, cnt = 0, top = 0;
         S s = 0;
-        uint64_t o = 0;
+        uint64_t o = 0, oend = 0;   // output cursor and end of the query's CSR segment
         while (true) {
             const bool need = !has;
             const uint64_t nm = __ballot(need);
@@ -330,7 +371,7 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
                     cur = pv.idx[k];
                     cnt = 0;
                     has = true;
-                    if (MODE != W_COUNT) o = offsets[q];
+                    if (MODE != W_COUNT) { o = offsets[q]; oend = offsets[q + 1]; }
                     if (MODE == W_VECTOR) { top = cur; lead = true; }
                 }
                 const uint32_t take = min((uint32_t)__popcll(nm), pool_left);
@@ -368,7 +409,7 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
                         lo = cur + 1u;
                     }
                     if (inv != 0 || cur == NONE) {
-                        c_emit = top - lo + 1u;
+                        c_emit = (uint32_t)min((uint64_t)(top - lo + 1u), oend - o);   // never past the segment
                         e_base = lo;
                         e_asc = true;
                         e_o = o;
@@ -379,7 +420,7 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
                     if (MODE == W_COUNT) {
                         cnt += (uint32_t)__popc(m);
                     } else {
-                        c_emit = (uint32_t)__popc(m);
+                        c_emit = (uint32_t)min((uint64_t)__popc(m), oend - o);   // never past the segment
                         e_m = m;
                         e_base = b0;
                         e_o = o;
@@ -499,19 +540,45 @@ __global__ __launch_bounds__(Q_THREADS) void k_coverage_i32(IndexView<int32_t> i
                    : [&] { using S = int64_t; return __VA_ARGS__; }())
 
 // Queries per tile and resident workgroups of the persistent walk kernel.
-constexpr uint32_t TILE = 2048;
+// Launch choices of the walk kernels, set through siTuneSet (the autotuner's knobs).
+//   tile         sorted queries per workgroup tile
+//   count_wgs    resident workgroups per CU for the count pass (0 = occupancy limit)
+//   fill_wgs     resident workgroups per CU for the fill passes (0 = occupancy limit)
+//   count_rank   siCount / siCountPlanned use the rank form instead of the walk
+//   offsets_rank siSearchOffsets* use the rank form (default) instead of the walk
+struct Tuning {
+    uint32_t tile = 2048;
+    int count_wgs = 0;
+    int fill_wgs = 0;
+    int count_rank = 0;
+    int offsets_rank = 1;
+};
+Tuning g_tune;
+
+template <typename S>
+int launch_rank(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint64_t* counts64,
+                hipStream_t st) {
+    if (nq == 0) return SI_OK;
+    k_count_rank<S><<<grid_for(nq, Q_THREADS), Q_THREADS, 0, st>>>(
+        ix->view<S>(), (const S*)ix->d_ends_sorted, plan_view<S>(plan, nq), nq, counts, counts64);
+    LAUNCH_CHECK();
+    return SI_OK;
+}
 
 template <typename S, int MODE>
 unsigned walk_grid(uint32_t ntiles) {
-    static unsigned blocks_full = 0;
-    if (blocks_full == 0) {
-        int dev = 0, cus = 0, per = 0;
+    static int cus = 0, per = 0;
+    if (cus == 0) {
+        int dev = 0;
         hipGetDevice(&dev);
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_walk<S, MODE>, T_THREADS, 0);
-        blocks_full = (unsigned)std::max(1, cus * std::max(per, 1));
+        cus = std::max(cus, 1);
+        per = std::max(per, 1);
     }
-    return std::max(1u, std::min(blocks_full, ntiles));
+    const int want = (MODE == W_COUNT) ? g_tune.count_wgs : g_tune.fill_wgs;
+    const int wgs = (want > 0) ? std::min(want, per) : per;
+    return std::max(1u, std::min((unsigned)(cus * wgs), ntiles));
 }
 
 template <typename S, int MODE>
@@ -519,11 +586,12 @@ int launch_walk(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint
                 const uint64_t* offsets, uint32_t* out_idx, int32_t* out_val, hipStream_t st) {
     if (nq == 0) return SI_OK;
     PlanView<S> pv = plan_view<S>(plan, nq);
-    const uint32_t ntiles = (uint32_t)((nq + TILE - 1) / TILE);
+    const uint32_t tile = g_tune.tile;
+    const uint32_t ntiles = (uint32_t)((nq + tile - 1) / tile);
     k_zero_u32<<<1, 64, 0, st>>>(pv.counter, 1);
     LAUNCH_CHECK();
     k_walk<S, MODE><<<walk_grid<S, MODE>(ntiles), T_THREADS, 0, st>>>(
-        ix->view<S>(), pv, (uint32_t)nq, TILE, ntiles, pv.counter, counts, counts64, offsets, out_idx,
+        ix->view<S>(), pv, (uint32_t)nq, tile, ntiles, pv.counter, counts, counts64, offsets, out_idx,
         ix->d_values, out_val);
     LAUNCH_CHECK();
     return SI_OK;
@@ -553,6 +621,41 @@ struct ScopedPlan {
 }  // namespace
 
 // ============================================================================
+// Tuning
+// ============================================================================
+SI_EXPORT int siTuneSet(const char* name, int64_t value) {
+    if (!name) return si_set_error(SI_ERR_ARG, "null tuning name");
+    const std::string k(name);
+    if (k == "tile") {
+        if (value < 64 || value > (1 << 20)) return si_set_error(SI_ERR_ARG, "tile must be in [64, 2^20]");
+        g_tune.tile = (uint32_t)value;
+    } else if (k == "count_wgs") {
+        if (value < 0 || value > 64) return si_set_error(SI_ERR_ARG, "count_wgs must be in [0, 64]");
+        g_tune.count_wgs = (int)value;
+    } else if (k == "fill_wgs") {
+        if (value < 0 || value > 64) return si_set_error(SI_ERR_ARG, "fill_wgs must be in [0, 64]");
+        g_tune.fill_wgs = (int)value;
+    } else if (k == "count_rank" || k == "offsets_rank") {
+        if (value != 0 && value != 1) return si_set_error(SI_ERR_ARG, "%s must be 0 or 1", name);
+        (k == "count_rank" ? g_tune.count_rank : g_tune.offsets_rank) = (int)value;
+    } else {
+        return si_set_error(SI_ERR_ARG, "unknown tuning parameter '%s'", name);
+    }
+    return SI_OK;
+}
+
+SI_EXPORT int64_t siTuneGet(const char* name) {
+    if (!name) return -1;
+    const std::string k(name);
+    if (k == "tile") return g_tune.tile;
+    if (k == "count_wgs") return g_tune.count_wgs;
+    if (k == "fill_wgs") return g_tune.fill_wgs;
+    if (k == "count_rank") return g_tune.count_rank;
+    if (k == "offsets_rank") return g_tune.offsets_rank;
+    return -1;
+}
+
+// ============================================================================
 // Planned batch entry points
 // ============================================================================
 SI_EXPORT size_t siPlanBytes(const siIndex* ix, size_t nq) {
@@ -576,6 +679,8 @@ SI_EXPORT int siCountPlanned(const siIndex* ix, void* plan, size_t nq, uint32_t*
     if (!plan) return si_set_error(SI_ERR_ARG, "null plan");
     if (nq && !counts) return si_set_error(SI_ERR_ARG, "null counts");
     hipStream_t st = (hipStream_t)stream;
+    if (g_tune.count_rank)
+        return DISPATCH_S(ix, (launch_rank<S>(ix, plan, nq, counts, nullptr, st)));
     return DISPATCH_S(ix, (launch_walk<S, W_COUNT>(ix, plan, nq, counts, nullptr, nullptr, nullptr, nullptr, st)));
 }
 
@@ -585,7 +690,10 @@ SI_EXPORT int siSearchOffsetsPlanned(const siIndex* ix, void* plan, size_t nq, u
     if (rc) return rc;
     if (!plan || !offsets) return si_set_error(SI_ERR_ARG, "null plan/offsets");
     hipStream_t st = (hipStream_t)stream;
-    rc = DISPATCH_S(ix, (launch_walk<S, W_COUNT>(ix, plan, nq, counts, offsets, nullptr, nullptr, nullptr, st)));
+    if (g_tune.offsets_rank)
+        rc = DISPATCH_S(ix, (launch_rank<S>(ix, plan, nq, counts, offsets, st)));
+    else
+        rc = DISPATCH_S(ix, (launch_walk<S, W_COUNT>(ix, plan, nq, counts, offsets, nullptr, nullptr, nullptr, st)));
     if (rc) return rc;
     uint64_t* ws = (uint64_t*)((char*)plan + PlanLayout(ix->cb, nq).off_scratch);
     return device_excl_scan<uint64_t, uint64_t>(offsets, nq, offsets, offsets + nq, ws, st);

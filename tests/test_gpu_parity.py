@@ -102,7 +102,14 @@ def test_random_vs_oracle(sia, orc, seed, dtype):
     assert np.array_equal(a["starts"], oix["starts"]) and np.array_equal(a["ends"], oix["ends"])
     assert np.array_equal(a["branch"], oix["branch"])
     assert np.array_equal(a["values"], oix["data"])
-    assert np.array_equal(ix.count(qs, qe), orc.count_batch(oix, qs, qe))
+    ocount = orc.count_batch(oix, qs, qe)
+    assert np.array_equal(ix.count(qs, qe), ocount)
+    from superintervals_amd import autotune
+    autotune.apply({"count_rank": 1})       # rank form, walk fallback for inverted queries
+    try:
+        assert np.array_equal(ix.count(qs, qe), ocount)
+    finally:
+        autotune.apply({"count_rank": 0})
     for order, mode in (("desc", 0), ("vector", 1)):
         off, idx = ix.search(qs, qe, order=order)
         ooff, oidx = orc.search_batch(oix, qs, qe, mode=mode)
@@ -155,6 +162,28 @@ def test_planned_batch_reuse(sia, orc, dtype):
     ix.values_fill_planned(plan, nq, off, vals)
     ooff, oidx = orc.search_batch(oix, qs, qe, mode=0)
     assert np.array_equal(vals.cpu().numpy(), oix["data"][oidx])
+    ix.close()
+
+
+def test_launch_choices_do_not_change_results(sia, orc):
+    """Every tile size / residency choice the autotuner may pick gives identical output."""
+    from superintervals_amd import autotune
+    rec, s, e, qs, qe = load_golden("c2_1m")
+    ix = _dev_index(sia, s, e)
+    saved = autotune.current()
+    try:
+        for tile, cw, fw, cr, orank in ((64, 1, 1, 0, 1), (512, 0, 1, 1, 0), (4096, 1, 0, 1, 1),
+                                        (100000, 0, 0, 0, 0)):
+            print(f"[gpu] launch tile={tile} count_wgs={cw} fill_wgs={fw} rank={cr}/{orank}", flush=True)
+            autotune.apply({"tile": tile, "count_wgs": cw, "fill_wgs": fw, "count_rank": cr,
+                            "offsets_rank": orank})
+            assert np.array_equal(ix.count(qs, qe), rec["counts"])
+            off, idx = ix.search(qs, qe, order="vector")
+            assert orc.digest(orc.hash_csr(off, idx)) == int(rec["hash_vec_digest"])
+    finally:
+        autotune.apply(saved)
+    with pytest.raises(sia.SIError):
+        autotune.apply({"tile": 3})
     ix.close()
 
 
