@@ -48,20 +48,34 @@ enum WalkMode { W_COUNT = 0, W_DESC = 1, W_VECTOR = 2, W_VALUES = 3 };
 // ============================================================================
 // Plan
 // ============================================================================
+struct HeavyEntry;
+
 template <typename S>
 struct PlanView {
-    uint32_t* counter;   // work-stealing counter of the walk kernels
+    uint32_t* counter;   // [0] tiles taken, [1] heavy entries written, [2] heavy entries taken
     uint32_t* q;         // input query number of sorted position k
     S* s;                // query start of sorted position k
     S* e;                // query end of sorted position k
     uint32_t* idx;       // upper_bound of sorted position k
+    HeavyEntry* heavy;   // parked walks (see HeavyEntry)
 };
 
 // Byte layout of a plan of nq queries (coordinate width cb):
 //   [256 B counters][q: 4 nq][s: cb nq][e: cb nq][idx: 4 nq][scratch: keys 2 x 8 nq,
 //   vals 2 x 4 nq, radix temporaries]; the scratch is reused as the offsets scan workspace.
+// A query whose walk exceeds the per-lane step budget, parked for the wave-cooperative
+// kernel: sorted position, current position, leading-run top and phase, and the count so
+// far (count mode) or the next output offset (fill modes).
+struct HeavyEntry {
+    uint32_t k;
+    uint32_t cur;
+    uint32_t top;
+    uint32_t lead;
+    uint64_t acc;
+};
+
 struct PlanLayout {
-    size_t off_q, off_s, off_e, off_idx, off_scratch, off_k0, off_k1, off_v0, off_v1, off_rs, total;
+    size_t off_q, off_s, off_e, off_idx, off_scratch, off_k0, off_k1, off_v0, off_v1, off_rs, off_heavy, total;
     PlanLayout(int cb, size_t nq) {
         off_q = 256;
         off_s = off_q + align256(4 * nq);
@@ -74,7 +88,10 @@ struct PlanLayout {
         off_v1 = off_v0 + align256(4 * nq);
         off_rs = off_v1 + align256(4 * nq);
         size_t scan = align256(scan_ws_items(nq) * 8);
-        total = std::max(off_rs + rs_temp_bytes(nq), off_scratch + scan);
+        // heavy list: after the scan workspace; dead once a walk launch has finished
+        off_heavy = off_scratch + scan;
+        total = std::max(std::max(off_rs + rs_temp_bytes(nq), off_scratch + scan),
+                         off_heavy + align256(sizeof(HeavyEntry) * nq));
     }
 };
 
@@ -88,6 +105,7 @@ PlanView<S> plan_view(void* base, size_t nq) {
     v.s = (S*)(p + L.off_s);
     v.e = (S*)(p + L.off_e);
     v.idx = (uint32_t*)(p + L.off_idx);
+    v.heavy = (HeavyEntry*)(p + L.off_heavy);
     return v;
 }
 
@@ -303,7 +321,7 @@ constexpr int T_THREADS = 512;   // 8 waves per workgroup, two workgroups per CU
 
 template <typename S, int MODE>
 __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S> pv, uint32_t nq,
-                                                    uint32_t tile, uint32_t ntiles,
+                                                    uint32_t tile, uint32_t ntiles, uint32_t lane_steps,
                                                     uint32_t* __restrict__ tile_counter,
                                                     uint32_t* __restrict__ counts,
                                                     uint64_t* __restrict__ counts64,
@@ -349,7 +367,7 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
         // ---- walk the tile; each wave refills idle lanes from pools of sorted queries
         uint32_t pool_base = 0, pool_left = 0;
         bool exhausted = false, has = false, lead = false;
-        uint32_t q = 0, cur = NONE, cnt = 0, top = 0;
+        uint32_t q = 0, cur = NONE, cnt = 0, top = 0, kq = 0, steps = 0;
         S s = 0;
         uint64_t o = 0, oend = 0;   // output cursor and end of the query's CSR segment
         while (true) {
@@ -370,6 +388,8 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
                     s = pv.s[k];
                     cur = pv.idx[k];
                     cnt = 0;
+                    kq = k;
+                    steps = 0;
                     has = true;
                     if (MODE != W_COUNT) { o = offsets[q]; oend = offsets[q + 1]; }
                     if (MODE == W_VECTOR) { top = cur; lead = true; }
@@ -451,9 +471,108 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
                 }
                 lead = false;
                 has = false;
+            } else if (has && ++steps >= lane_steps) {
+                // over budget: park the walk for the wave-cooperative kernel, take another query
+                const uint32_t h = atomicAdd(&pv.counter[1], 1u);
+                HeavyEntry he;
+                he.k = kq;
+                he.cur = cur;
+                he.top = top;
+                he.lead = lead ? 1u : 0u;
+                he.acc = (MODE == W_COUNT) ? (uint64_t)cnt : o;
+                pv.heavy[h] = he;
+                lead = false;
+                has = false;
             }
         }
         __syncthreads();   // LDS window and counters are reused by the next tile
+    }
+}
+
+// Wave-cooperative continuation of parked walks: one wave per query, one aligned line of
+// 64 ends per step (a coalesced 256-byte load for int32), hits by ballot.  The line exit
+// is the same rule as the lane walk's, with the leftmost maximum found by a wave max
+// reduction and a ballot.  Output is written at each hit's rank among the line's hits.
+template <typename S, int MODE>
+__global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanView<S> pv,
+                                                          uint32_t* __restrict__ counts,
+                                                          uint64_t* __restrict__ counts64,
+                                                          const uint64_t* __restrict__ offsets,
+                                                          uint32_t* __restrict__ out_idx,
+                                                          const int32_t* __restrict__ vals,
+                                                          int32_t* __restrict__ out_val) {
+    const int lane = lane_id();
+    const uint64_t le_mask = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);   // lanes <= this
+    const uint32_t nheavy = pv.counter[1];
+    while (true) {
+        uint32_t h = 0;
+        if (lane == 0) h = atomicAdd(&pv.counter[2], 1u);
+        h = __shfl(h, 0, WAVE);
+        if (h >= nheavy) break;
+        const HeavyEntry he = pv.heavy[h];
+        const uint32_t k = he.k;
+        const uint32_t q = pv.q[k];
+        const S s = pv.s[k];
+        uint32_t cur = he.cur;
+        const uint32_t top = he.top;
+        bool lead = he.lead != 0;
+        uint64_t acc = he.acc;
+        const uint64_t oend = (MODE != W_COUNT) ? offsets[q + 1] : 0;
+        while (cur != NONE) {
+            const uint32_t b0 = cur & ~63u;
+            const uint32_t kc = cur - b0;
+            const S v = ix.ends[b0 + lane];
+            const bool hit = ((uint32_t)lane <= kc) && (v >= s);
+            const uint64_t m = __ballot(hit);
+            if (MODE == W_VECTOR && lead) {
+                const uint64_t lim = (kc == 63) ? ~0ull : ((2ull << kc) - 1ull);
+                const uint64_t inv = ~m & lim;
+                uint32_t lo;
+                if (inv == 0) {
+                    cur = b0 - 1u;
+                    lo = 0;
+                } else {
+                    const uint32_t hm = 63u - (uint32_t)__builtin_clzll(inv);
+                    cur = b0 + hm;
+                    lo = cur + 1u;
+                }
+                if (inv != 0 || cur == NONE) {
+                    const uint64_t c = min((uint64_t)(top - lo + 1u), oend - acc);
+                    for (uint64_t x = lane; x < c; x += WAVE) out_idx[acc + x] = lo + (uint32_t)x;
+                    acc += c;
+                    lead = false;
+                }
+                continue;
+            }
+            const uint32_t nh = (uint32_t)__popcll(m);
+            if (MODE == W_COUNT) {
+                acc += nh;
+            } else {
+                if (hit) {
+                    const uint64_t r = (uint64_t)__popcll(m & ~le_mask);   // hits above this lane
+                    if (acc + r < oend) {
+                        if (MODE == W_VALUES) out_val[acc + r] = vals[b0 + lane];
+                        else out_idx[acc + r] = b0 + (uint32_t)lane;
+                    }
+                }
+                acc = min(acc + nh, oend);
+            }
+            if (m & 1ull) {
+                cur = b0 - 1u;
+            } else {
+                const uint32_t x = m ? (uint32_t)(__builtin_ctzll(m) - 1) : kc;
+                const bool cand = (uint32_t)lane <= x;
+                S best = cand ? v : ((sizeof(S) == 4) ? (S)INT32_MIN : (S)INT64_MIN);
+                best = wave_max(best);
+                const uint64_t am = __ballot(cand && v == best);
+                const uint32_t a = (uint32_t)__builtin_ctzll(am);
+                cur = ix.branch[b0 + a];
+            }
+        }
+        if (MODE == W_COUNT && lane == 0) {
+            if (counts) counts[q] = (uint32_t)acc;
+            if (counts64) counts64[q] = acc;
+        }
     }
 }
 
@@ -546,12 +665,15 @@ __global__ __launch_bounds__(Q_THREADS) void k_coverage_i32(IndexView<int32_t> i
 //   fill_wgs     resident workgroups per CU for the fill passes (0 = occupancy limit)
 //   count_rank   siCount / siCountPlanned use the rank form instead of the walk
 //   offsets_rank siSearchOffsets* use the rank form (default) instead of the walk
+//   lane_steps   line steps a lane spends on one query before parking it for the
+//                wave-cooperative kernel
 struct Tuning {
     uint32_t tile = 2048;
     int count_wgs = 0;
     int fill_wgs = 0;
     int count_rank = 0;
     int offsets_rank = 1;
+    uint32_t lane_steps = 256;
 };
 Tuning g_tune;
 
@@ -588,11 +710,22 @@ int launch_walk(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint
     PlanView<S> pv = plan_view<S>(plan, nq);
     const uint32_t tile = g_tune.tile;
     const uint32_t ntiles = (uint32_t)((nq + tile - 1) / tile);
-    k_zero_u32<<<1, 64, 0, st>>>(pv.counter, 1);
+    k_zero_u32<<<1, 64, 0, st>>>(pv.counter, 3);
     LAUNCH_CHECK();
     k_walk<S, MODE><<<walk_grid<S, MODE>(ntiles), T_THREADS, 0, st>>>(
-        ix->view<S>(), pv, (uint32_t)nq, tile, ntiles, pv.counter, counts, counts64, offsets, out_idx,
-        ix->d_values, out_val);
+        ix->view<S>(), pv, (uint32_t)nq, tile, ntiles, g_tune.lane_steps, pv.counter, counts, counts64,
+        offsets, out_idx, ix->d_values, out_val);
+    LAUNCH_CHECK();
+    // parked (heavy) walks: one wave each, resident grid, exits at once when none were parked
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        cus = std::max(cus, 1);
+    }
+    k_walk_heavy<S, MODE><<<(unsigned)cus * 8u, Q_THREADS, 0, st>>>(ix->view<S>(), pv, counts, counts64, offsets,
+                                                                   out_idx, ix->d_values, out_val);
     LAUNCH_CHECK();
     return SI_OK;
 }
@@ -635,6 +768,9 @@ SI_EXPORT int siTuneSet(const char* name, int64_t value) {
     } else if (k == "fill_wgs") {
         if (value < 0 || value > 64) return si_set_error(SI_ERR_ARG, "fill_wgs must be in [0, 64]");
         g_tune.fill_wgs = (int)value;
+    } else if (k == "lane_steps") {
+        if (value < 1 || value > (1 << 30)) return si_set_error(SI_ERR_ARG, "lane_steps must be in [1, 2^30]");
+        g_tune.lane_steps = (uint32_t)value;
     } else if (k == "count_rank" || k == "offsets_rank") {
         if (value != 0 && value != 1) return si_set_error(SI_ERR_ARG, "%s must be 0 or 1", name);
         (k == "count_rank" ? g_tune.count_rank : g_tune.offsets_rank) = (int)value;
@@ -650,6 +786,7 @@ SI_EXPORT int64_t siTuneGet(const char* name) {
     if (k == "tile") return g_tune.tile;
     if (k == "count_wgs") return g_tune.count_wgs;
     if (k == "fill_wgs") return g_tune.fill_wgs;
+    if (k == "lane_steps") return g_tune.lane_steps;
     if (k == "count_rank") return g_tune.count_rank;
     if (k == "offsets_rank") return g_tune.offsets_rank;
     return -1;

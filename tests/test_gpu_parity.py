@@ -187,6 +187,53 @@ def test_launch_choices_do_not_change_results(sia, orc):
     ix.close()
 
 
+@pytest.mark.parametrize("lane_steps", [1, 3, 17])
+def test_heavy_path_parity(sia, orc, lane_steps):
+    """Walks parked after lane_steps lines finish in the wave-cooperative kernel with the
+    same results, in every mode (count by walk, descending, vector, values)."""
+    from superintervals_amd import autotune
+    print(f"[gpu] heavy path lane_steps={lane_steps}", flush=True)
+    rec, s, e, qs, qe = load_golden("c1_shuffled")
+    ix = _dev_index(sia, s, e, np.arange(s.size, dtype=np.int32))
+    oix = orc.build_index(s, e)
+    saved = autotune.current()
+    try:
+        autotune.apply({"lane_steps": lane_steps, "offsets_rank": 0, "count_rank": 0})
+        assert np.array_equal(ix.count(qs, qe), rec["counts"])
+        off, idx = ix.search(qs, qe, order="desc")
+        assert orc.digest(orc.hash_csr(off, idx)) == int(rec["hash_desc_digest"])
+        off, idx = ix.search(qs, qe, order="vector")
+        assert orc.digest(orc.hash_csr(off, idx)) == int(rec["hash_vec_digest"])
+        off, vals = ix.search(qs, qe, values=True)
+        assert orc.digest(orc.hash_csr(off, vals)) == int(rec["hash_cvalues_digest"])
+    finally:
+        autotune.apply(saved)
+    ix.close()
+
+
+def test_workload3_wide_queries(sia, orc):
+    """int64 hotspot mixture with 1 % wide queries (workload 3 distributions, 2M intervals),
+    unsorted input: full CSR against the oracle in both orders."""
+    print("[gpu] workload 3 (2M int64 intervals, 20K queries)", flush=True)
+    s, e = sia.workload_intervals(3, 2_000_000, seed=31, sorted=False)
+    qs, qe = sia.workload_queries(3, 20_000, seed=32)
+    ix = _dev_index(sia, s, e, np.arange(s.size, dtype=np.int32))
+    oix = orc.build_index(s, e)
+    a = ix.download(values=True)
+    assert np.array_equal(a["starts"], oix["starts"]) and np.array_equal(a["branch"], oix["branch"])
+    assert np.array_equal(a["values"], oix["data"])
+    counts = ix.count(qs, qe)
+    ocounts = orc.count_batch(oix, qs, qe)
+    assert np.array_equal(counts, ocounts)
+    assert ocounts.max() > 5_000           # the wide queries exceed the lane step budget
+    for order, mode in (("desc", 0), ("vector", 1)):
+        off, idx = ix.search(qs, qe, order=order)
+        ooff, oidx = orc.search_batch(oix, qs, qe, mode=mode)
+        assert np.array_equal(off, ooff)
+        assert np.array_equal(idx.astype(np.int64), oidx)
+    ix.close()
+
+
 def test_walk_stats_match_oracle(sia, orc):
     import torch
     rec, s, e, qs, qe = load_golden("c1_sorted")
