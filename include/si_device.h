@@ -202,6 +202,45 @@ int siCoverage(const siIndex* ix, const void* qs, const void* qe, size_t nq, uin
 int siWalkStats(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint64_t* totals,
                 void* stream);
 
+/* ---- Learned position index (no counterpart in the reference) ---------------
+ * A two-stage FP32 model of rank(start): a root MLP 1 -> 16 -> 16 -> 1 that picks one of
+ * `leaves` linear models, each with the exact min / max error over the keys it serves.
+ * siUpperBoundLearned searches only the predicted window, checks the result against the
+ * neighbours outside it and falls back to the plain binary search when the check fails,
+ * so its output always equals siUpperBound's. */
+typedef struct siLearned siLearned;
+
+/* Attach an untrained model to a built index (the index must outlive the model). */
+siLearned* siLearnedCreate(const siIndex* ix, uint32_t leaves);
+void siLearnedDestroy(siLearned* m);
+
+/* Train on the GPU: `steps` Adam steps of minibatch MSE (batch samples of the index's own
+ * (start, position / n) pairs, drawn deterministically), then fit the leaves and their
+ * error bounds.  steps = 0 keeps the identity-CDF initialisation (routing by key alone).
+ * Synchronises the stream. */
+int siLearnedTrain(siLearned* m, int steps, uint32_t batch, float lr, void* stream);
+
+/* Number of root parameters (W1[16], b1[16], W2[16][16], b2[16], w3[16], b3). */
+int siLearnedParamCount(void);
+int siLearnedGetParams(const siLearned* m, float* params);
+
+/* Normalisation of keys: x = (key - kmin) * inv_span. */
+int siLearnedNorm(const siLearned* m, int64_t* kmin, float* inv_span);
+
+/* max_abs_err: largest |position error| of any leaf over its keys; last_loss: final
+ * minibatch loss; used_leaves: leaves serving at least one key.  Each may be NULL. */
+int siLearnedInfo(const siLearned* m, int64_t* max_abs_err, float* last_loss, uint32_t* used_leaves);
+
+/* Training-loss gradient at params (host, NULL = the model's) over the given sample
+ * positions, computed by the training kernels; grad receives siLearnedParamCount() + 1
+ * floats, the last being the loss. */
+int siLearnedGradient(siLearned* m, const float* params, const uint32_t* pos, uint32_t count, float* grad);
+
+/* upper_bound through the model, device buffers; identical to siUpperBound.  stats (device,
+ * may be NULL) accumulates [0] fallbacks to the plain search and [1] total window width. */
+int siUpperBoundLearned(const siLearned* m, const void* xs, size_t n, uint32_t* out, uint64_t* stats,
+                        void* stream);
+
 /* ---- Host-buffer convenience forms (copy in, compute, copy out, synchronise) -- */
 
 int siCountHost(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint32_t* counts);

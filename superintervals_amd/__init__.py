@@ -20,7 +20,7 @@ import numpy as np
 from . import _native as N
 from ._native import SI_ORDER_DESC, SI_ORDER_VECTOR, SI_INDEX_NONE, SIError
 
-__all__ = ["DeviceIndex", "IntervalMap", "SIError", "SI_ORDER_DESC", "SI_ORDER_VECTOR",
+__all__ = ["DeviceIndex", "LearnedIndex", "IntervalMap", "SIError", "SI_ORDER_DESC", "SI_ORDER_VECTOR",
            "SI_INDEX_NONE", "workload_intervals", "workload_queries", "add_flags", "lib_path"]
 
 lib_path = N.LIB_PATH
@@ -271,6 +271,63 @@ class DeviceIndex:
     def values_fill_planned(self, plan_t, nq, offsets_t, vals_t, stream=None):
         N.check(N.lib.siSearchValuesFillPlanned(self._h, _ptr(plan_t), nq, _ptr(offsets_t), _ptr(vals_t),
                                                 _stream_of(stream)), "siSearchValuesFillPlanned")
+
+
+class LearnedIndex:
+    """Learned position model attached to a DeviceIndex (si_device.h siLearned)."""
+
+    def __init__(self, index: DeviceIndex, leaves: int = 4096):
+        h = N.lib.siLearnedCreate(index.handle, leaves)
+        if not h:
+            raise SIError(N.SI_ERR_ARG, "siLearnedCreate")
+        self._h = h
+        self.index = index        # keeps the index alive
+        self.leaves = leaves
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.lib.siLearnedDestroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def train(self, steps: int = 400, batch: int = 65536, lr: float = 3e-3, stream=None):
+        N.check(N.lib.siLearnedTrain(self._h, steps, batch, lr, _stream_of(stream)), "siLearnedTrain")
+        return self.info()
+
+    @staticmethod
+    def param_count() -> int:
+        return int(N.lib.siLearnedParamCount())
+
+    def params(self) -> np.ndarray:
+        p = np.empty(self.param_count(), np.float32)
+        N.check(N.lib.siLearnedGetParams(self._h, _ptr(p)), "siLearnedGetParams")
+        return p
+
+    def norm(self):
+        kmin = C.c_int64(0)
+        inv = C.c_float(0)
+        N.check(N.lib.siLearnedNorm(self._h, C.byref(kmin), C.byref(inv)), "siLearnedNorm")
+        return int(kmin.value), float(inv.value)
+
+    def info(self) -> dict:
+        e = C.c_int64(0)
+        loss = C.c_float(0)
+        used = C.c_uint32(0)
+        N.check(N.lib.siLearnedInfo(self._h, C.byref(e), C.byref(loss), C.byref(used)), "siLearnedInfo")
+        return {"max_abs_err": int(e.value), "loss": float(loss.value), "used_leaves": int(used.value)}
+
+    def gradient(self, positions, params=None) -> np.ndarray:
+        pos = np.ascontiguousarray(positions, np.uint32)
+        out = np.empty(self.param_count() + 1, np.float32)
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        N.check(N.lib.siLearnedGradient(self._h, _ptr(p), _ptr(pos), pos.size, _ptr(out)), "siLearnedGradient")
+        return out
+
+    def upper_bound_device(self, xs_t, out_t, stats_t=None, stream=None):
+        N.check(N.lib.siUpperBoundLearned(self._h, _ptr(xs_t), xs_t.numel(), _ptr(out_t), _ptr(stats_t),
+                                          _stream_of(stream)), "siUpperBoundLearned")
 
 
 class IntervalMap:

@@ -55,6 +55,15 @@ _SIGS = {
     "siSearchValuesFill": (_i, [_vp, _vp, _vp, _sz, _vp, _vp, _vp]),
     "siCoverage": (_i, [_vp, _vp, _vp, _sz, _vp, _vp, _vp]),
     "siWalkStats": (_i, [_vp, _vp, _vp, _sz, _vp, _vp]),
+    "siLearnedCreate": (_vp, [_vp, C.c_uint32]),
+    "siLearnedDestroy": (None, [_vp]),
+    "siLearnedTrain": (_i, [_vp, _i, C.c_uint32, C.c_float, _vp]),
+    "siLearnedParamCount": (_i, []),
+    "siLearnedGetParams": (_i, [_vp, _vp]),
+    "siLearnedNorm": (_i, [_vp, _vp, _vp]),
+    "siLearnedInfo": (_i, [_vp, _vp, _vp, _vp]),
+    "siLearnedGradient": (_i, [_vp, _vp, _vp, C.c_uint32, _vp]),
+    "siUpperBoundLearned": (_i, [_vp, _vp, _sz, _vp, _vp, _vp]),
     "siTuneSet": (_i, [C.c_char_p, C.c_int64]),
     "siTuneGet": (C.c_int64, [C.c_char_p]),
     "siPlanBytes": (_sz, [_vp, _sz]),

@@ -234,6 +234,68 @@ def test_workload3_wide_queries(sia, orc):
     ix.close()
 
 
+def test_learned_index(sia, orc):
+    """Learned upper_bound (workload 4 distributions, 2M intervals): GPU gradients against the
+    float64 oracle, output identical to the plain search, and the trained root's gain over the
+    untrained (identity-CDF) one."""
+    import torch
+    from oracle import rmi
+    print("[gpu] learned index", flush=True)
+    s, e = sia.workload_intervals(4, 2_000_000, seed=41, sorted=False)
+    ix = _dev_index(sia, s, e)
+    starts = ix.download()["starts"]
+    n = starts.size
+    dev = torch.device("cuda:0")
+
+    untrained = sia.LearnedIndex(ix, 4096)
+    info0 = untrained.train(steps=0)
+    model = sia.LearnedIndex(ix, 4096)
+    info1 = model.train(steps=400, batch=65536, lr=3e-3)
+    assert info1["loss"] < 1e-3
+
+    # gradients of the training kernels against the oracle, at the initial and trained points
+    kmin, inv = model.norm()
+    pos = np.random.default_rng(3).integers(0, n, 4096).astype(np.uint32)
+    x = (np.float64(starts[pos].astype(np.int64) - kmin) * np.float64(np.float32(inv))).astype(np.float32)
+    t = (pos.astype(np.float32) / np.float32(n)).astype(np.float32)
+    for p in (rmi.identity_params().astype(np.float32), model.params()):
+        g_gpu = model.gradient(pos, p)
+        g_cpu, l_cpu = rmi.gradient(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64))
+        scale = np.abs(g_cpu).max()
+        assert np.all(np.abs(g_gpu[:-1] - g_cpu) <= 1e-4 * scale + 2e-3 * np.abs(g_cpu)), \
+            np.abs(g_gpu[:-1] - g_cpu).max() / scale
+        assert abs(g_gpu[-1] - l_cpu) <= 1e-5 + 1e-3 * l_cpu
+
+    # identical to the plain upper_bound on point, short-range and out-of-range queries
+    qs, qe = sia.workload_queries(4, 1_000_000, seed=42)
+    xs = np.concatenate([qe, qs, starts[:1000], starts[-1000:], np.array([np.iinfo(np.int32).min,
+                         np.iinfo(np.int32).max, int(starts[0]) - 1, int(starts[-1]) + 1], np.int32)])
+    xt = torch.from_numpy(xs).to(dev)
+    plain = torch.empty(xs.size, dtype=torch.int32, device=dev)
+    assert sia._native.lib.siUpperBound(ix.handle, xt.data_ptr(), xs.size, plain.data_ptr(), None) == 0
+    want = np.searchsorted(starts, xs, side="right").astype(np.int64) - 1
+    got_plain = plain.cpu().numpy().view(np.uint32).astype(np.int64)
+    got_plain[got_plain == 0xFFFFFFFF] = -1
+    assert np.array_equal(got_plain, want)
+    widths = {}
+    for name, m in (("untrained", untrained), ("trained", model)):
+        out = torch.empty(xs.size, dtype=torch.int32, device=dev)
+        st = torch.zeros(2, dtype=torch.int64, device=dev)
+        m.upper_bound_device(xt, out, st)
+        got = out.cpu().numpy().view(np.uint32).astype(np.int64)
+        got[got == 0xFFFFFFFF] = -1
+        assert np.array_equal(got, want), name
+        fb, w = st.cpu().tolist()
+        widths[name] = w / xs.size
+        print(f"[gpu] {name}: fallbacks {fb}, mean window {w / xs.size:.1f}", flush=True)
+    # what the trained network gains: narrower exact windows than routing by key alone
+    assert info1["max_abs_err"] < info0["max_abs_err"]
+    assert widths["trained"] < widths["untrained"]
+    model.close()
+    untrained.close()
+    ix.close()
+
+
 def test_walk_stats_match_oracle(sia, orc):
     import torch
     rec, s, e, qs, qe = load_golden("c1_sorted")

@@ -258,3 +258,27 @@ def test_reference_build_cross_check(orc):
         assert m.count(qs[q], qe[q]) == orc.count(ix, qs[q], qe[q])
         assert np.array_equal(m.search(qs[q], qe[q], 1), orc.search(ix, qs[q], qe[q], 1))
         assert np.array_equal(c.search(qs[q], qe[q], 0), orc.search(ix, qs[q], qe[q], 0))
+
+
+# ---- learned-index model oracle (oracle/rmi.py), pinned by finite differences ----------
+def test_rmi_oracle_gradient_finite_differences():
+    from oracle import rmi
+    rng = np.random.default_rng(4)
+    x = rng.random(300)
+    t = np.sort(rng.random(300))
+    for p in (rmi.identity_params(), rmi.identity_params() + 0.05 * rng.standard_normal(rmi.NPARAM)):
+        g, l0 = rmi.gradient(p, x, t)
+        assert abs(l0 - rmi.loss(p, x, t)) < 1e-14
+        h = 1e-6
+        for k in rng.choice(rmi.NPARAM, 60, replace=False).tolist() + [0, 16, 32, 288, 304, 320]:
+            e = np.zeros(rmi.NPARAM)
+            e[k] = h
+            fd = (rmi.loss(p + e, x, t) - rmi.loss(p - e, x, t)) / (2 * h)
+            assert abs(fd - g[k]) <= 1e-6 + 1e-4 * abs(fd), (k, fd, g[k])
+
+
+def test_rmi_identity_init_is_identity_cdf():
+    from oracle import rmi
+    x = np.linspace(0, 1, 101)
+    assert np.allclose(rmi.forward(rmi.identity_params(), x), x)
+    assert rmi.leaf_of(np.array([-0.5, 0.0, 0.5, 0.99999, 1.5, np.nan]), 4096).tolist() == [0, 0, 2048, 4095, 4095, 0]
