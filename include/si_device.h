@@ -118,6 +118,8 @@ double siIndexLastBuildMs(const siIndex* ix);
  *   "count_rank"   1: siCount* use the rank form (upper_bound + 1 - #{ends < s}, exact
  *                  for s <= e; inverted queries walk) instead of the walk (default 0)
  *   "offsets_rank" 1 (default): siSearchOffsets* count with the rank form; 0: the walk
+ *   "lane_steps"   line steps a lane spends on one query before the walk is parked for
+ *                  the wave-cooperative kernel (1 .. 2^30, default 256)
  * superintervals_amd/autotune.py measures candidates and stores the best per workload. */
 int siTuneSet(const char* name, int64_t value);
 int64_t siTuneGet(const char* name);   /* -1 for an unknown name */

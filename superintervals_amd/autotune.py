@@ -17,7 +17,7 @@ import time
 from . import _native as N
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "superintervals_amd", "tune.json")
-KNOBS = ("tile", "count_wgs", "fill_wgs", "count_rank", "offsets_rank")
+KNOBS = ("tile", "count_wgs", "fill_wgs", "count_rank", "offsets_rank", "lane_steps")
 CANDIDATE_TILES = (512, 1024, 2048, 4096)
 CANDIDATE_WGS = (1, 0)     # 0 = the occupancy limit
 
@@ -26,7 +26,15 @@ def current() -> dict:
     return {k: int(N.lib.siTuneGet(k.encode())) for k in KNOBS}
 
 
+def knobs_of(record: dict) -> dict:
+    """The knob entries of a tuning record (which also holds measured times)."""
+    return {k: record[k] for k in KNOBS if k in record}
+
+
 def apply(params: dict) -> None:
+    unknown = set(params) - set(KNOBS)
+    if unknown:
+        raise KeyError("unknown tuning knob(s): %s" % ", ".join(sorted(unknown)))
     for k in KNOBS:
         if k in params:
             N.check(N.lib.siTuneSet(k.encode(), int(params[k])), f"siTuneSet({k})")
@@ -101,7 +109,7 @@ def tune(ix, plan_t, nq, offsets_t, idx_t, stream=None, reps=3, tiles=CANDIDATE_
             "fill_wgs": by_tile[tile]["fill"]["fill_wgs"],
             "count_ms": by_tile[tile]["count"]["count_ms"], "fill_ms": by_tile[tile]["fill"]["fill_ms"]}
     del best_count
-    apply(best)
+    apply(knobs_of(best))
     return best, results
 
 
@@ -111,7 +119,7 @@ def load_or_tune(ix, plan_t, nq, offsets_t, idx_t, hits, cache=DEFAULT_CACHE, st
     sig = signature(ix.dtype.itemsize, ix.n, nq, hits)
     table = _load(cache)
     if not retune and sig in table:
-        apply(table[sig])
+        apply(knobs_of(table[sig]))
         return dict(table[sig], cached=True, signature=sig)
     t0 = time.perf_counter()
     best, _ = tune(ix, plan_t, nq, offsets_t, idx_t, stream=stream, verbose=verbose)

@@ -377,7 +377,7 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
                 if (pool_left == 0 && !exhausted) {
                     uint32_t b = 0;
                     if (lane == 0) b = atomicAdd(&s_next, POOL);
-                    b = __shfl(b, 0, WAVE);
+                    b = __builtin_amdgcn_readfirstlane(__shfl(b, 0, WAVE));   // pool state is wave-uniform
                     if (b >= t1) exhausted = true;
                     else { pool_base = b; pool_left = min(POOL, t1 - b); }
                 }
@@ -493,6 +493,13 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
 // 64 ends per step (a coalesced 256-byte load for int32), hits by ballot.  The line exit
 // is the same rule as the lane walk's, with the leftmost maximum found by a wave max
 // reduction and a ballot.  Output is written at each hit's rank among the line's hits.
+//
+// All 64 lanes must reach every ballot and shuffle together.  The values that steer the
+// loops (entry number, walk position) are therefore made wave-uniform with readfirstlane,
+// and the loop holds a single lane-dependent branch (lane 0's atomic).  With a second
+// lane-0-only block (a lane-0 store at the end of the body) the compiler merged the two
+// lane == 0 tests and sent lanes 1..63 back to the shuffle without lane 0; they read the
+// disabled lane, re-walked entry 0 and took an empty ballot's -1 as a line offset.
 template <typename S, int MODE>
 __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanView<S> pv,
                                                           uint32_t* __restrict__ counts,
@@ -507,15 +514,15 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanV
     while (true) {
         uint32_t h = 0;
         if (lane == 0) h = atomicAdd(&pv.counter[2], 1u);
-        h = __shfl(h, 0, WAVE);
+        h = __builtin_amdgcn_readfirstlane(__shfl(h, 0, WAVE));
         if (h >= nheavy) break;
         const HeavyEntry he = pv.heavy[h];
-        const uint32_t k = he.k;
-        const uint32_t q = pv.q[k];
+        const uint32_t k = __builtin_amdgcn_readfirstlane(he.k);
+        const uint32_t q = __builtin_amdgcn_readfirstlane(pv.q[k]);
         const S s = pv.s[k];
-        uint32_t cur = he.cur;
-        const uint32_t top = he.top;
-        bool lead = he.lead != 0;
+        uint32_t cur = __builtin_amdgcn_readfirstlane(he.cur);
+        const uint32_t top = __builtin_amdgcn_readfirstlane(he.top);
+        bool lead = __builtin_amdgcn_readfirstlane(he.lead) != 0;
         uint64_t acc = he.acc;
         const uint64_t oend = (MODE != W_COUNT) ? offsets[q + 1] : 0;
         while (cur != NONE) {
@@ -565,11 +572,14 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanV
                 S best = cand ? v : ((sizeof(S) == 4) ? (S)INT32_MIN : (S)INT64_MIN);
                 best = wave_max(best);
                 const uint64_t am = __ballot(cand && v == best);
-                const uint32_t a = (uint32_t)__builtin_ctzll(am);
-                cur = ix.branch[b0 + a];
+                // lane 0 is a candidate whenever b0 is a miss, so am != 0 with all lanes
+                // active; the guard keeps the offset inside the line regardless
+                const uint32_t a = am ? (uint32_t)__builtin_ctzll(am) : 0u;
+                cur = __builtin_amdgcn_readfirstlane(ix.branch[b0 + a]);
             }
         }
-        if (MODE == W_COUNT && lane == 0) {
+        if (MODE == W_COUNT) {
+            // every lane stores the same value (no lane-dependent branch in the loop)
             if (counts) counts[q] = (uint32_t)acc;
             if (counts64) counts64[q] = acc;
         }

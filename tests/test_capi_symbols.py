@@ -45,3 +45,22 @@ def test_no_gpu_errors_cleanly():
     h = N.lib.siIndexCreate(4)
     assert not h
     assert b"device" in N.lib.siGetLastError()
+
+
+def test_tuning_knobs_round_trip():
+    """Every knob the autotuner saves and restores reaches the library (host-side state,
+    no GPU call), and an unknown knob name is an error rather than silently ignored."""
+    import pytest
+    from superintervals_amd import autotune
+    saved = autotune.current()
+    assert set(saved) == set(autotune.KNOBS)
+    assert "lane_steps" in saved and saved["lane_steps"] >= 1
+    try:
+        autotune.apply({"lane_steps": 3, "tile": 512})
+        now = autotune.current()
+        assert now["lane_steps"] == 3 and now["tile"] == 512
+        with pytest.raises(KeyError):
+            autotune.apply({"lane_step": 3})
+    finally:
+        autotune.apply(saved)
+    assert autotune.current() == saved
