@@ -264,47 +264,79 @@ __device__ __forceinline__ uint32_t exit_pos(const Line<S>& L, uint32_t kc, uint
     return a;
 }
 
+// One slot of a wave's emission scratch in LDS: the segment that starts at this output
+// slot of the current 64-slot chunk.  kv = m | kind << 16 | 1 << 31 (valid); 0 when free.
+struct __align__(8) EmitSlot {
+    uint32_t pre, kv, base, c;
+    uint32_t o_lo, o_hi;
+};
+
+// Orders this wave's LDS accesses for the compiler; LDS executes a wave's accesses in
+// issue order, so no wait is needed between lanes of one wave.
+__device__ __forceinline__ void wave_lds_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
 // Wave-cooperative output.  Every lane holds this step's segment (c values at out
 // offset o); the wave scans the counts and lane x writes element x of the concatenated
 // segments, so one store instruction covers 64 consecutive output slots of a few lists
-// instead of one slot in each of 64 lists.  All lanes execute the shuffles.
+// instead of one slot in each of 64 lists.  The slot -> segment map of each 64-slot
+// chunk goes through the wave's 64 LDS slots (sd): a lane whose segment starts in the
+// chunk writes its descriptor at its start slot, a ballot gives the chunk's start mask,
+// and slot x takes the last start at or before x (or the segment running in from the
+// previous chunk).  A store happens only at r < c of its segment, so no slot is written
+// outside the segment its owner was given.  All lanes execute the ballots.
 template <typename S, int MODE>
 __device__ __forceinline__ void emit_segments(uint32_t c, uint32_t m, uint32_t base, bool asc,
-                                              uint64_t o, int lane, uint32_t* __restrict__ out_idx,
+                                              uint64_t o, int lane, EmitSlot* __restrict__ sd,
+                                              uint32_t* __restrict__ out_idx,
                                               const int32_t* __restrict__ vals,
                                               int32_t* __restrict__ out_val) {
     constexpr int W = Line<S>::W;
+    if (__ballot(c != 0) == 0) return;
     const uint32_t inc = wave_incl_scan(c);
-    const uint32_t total = __shfl(inc, WAVE - 1, WAVE);
-    if (total == 0) return;
+    const uint32_t total = __builtin_amdgcn_readlane(inc, WAVE - 1);
     const uint32_t pre = inc - c;
-    const uint32_t flag = asc ? 1u : 0u;
+    const uint64_t le = (lane == WAVE - 1) ? ~0ull : ((2ull << lane) - 1ull);   // lanes <= this
+    EmitSlot cy = {0u, 0u, 0u, 0u, 0u, 0u};
     for (uint32_t x0 = 0; x0 < total; x0 += WAVE) {
-        const uint32_t x = x0 + (uint32_t)lane;
-        // owner: the largest lane l with pre_l <= x (it has c_l > 0 whenever x < total)
-        int l = 0;
-#pragma unroll
-        for (int stp = 32; stp >= 1; stp >>= 1) {
-            const uint32_t pc = __shfl(pre, l + stp, WAVE);
-            l = (pc <= x) ? l + stp : l;
+        if (c != 0 && pre >= x0 && pre - x0 < (uint32_t)WAVE) {
+            EmitSlot e;
+            e.pre = pre;
+            e.kv = m | ((asc ? 1u : 0u) << 16) | 0x80000000u;
+            e.base = base;
+            e.c = c;
+            e.o_lo = (uint32_t)o;
+            e.o_hi = (uint32_t)(o >> 32);
+            sd[pre - x0] = e;
         }
-        const uint32_t r = x - __shfl(pre, l, WAVE);
-        const uint32_t om = __shfl(m, l, WAVE);
-        const uint32_t ob = __shfl(base, l, WAVE);
-        const uint32_t of = __shfl(flag, l, WAVE);
-        const uint64_t oo = __shfl(o, l, WAVE);
+        wave_lds_sync();
+        const uint64_t mine = __ballot((sd[lane].kv >> 31) != 0u) & le;
+        EmitSlot d = cy;
+        if (mine) d = sd[63u - (uint32_t)__builtin_clzll(mine)];
+        wave_lds_sync();
+        sd[lane].kv = 0u;   // free the slot for the next chunk (every read above is done)
+        // the segment holding the chunk's last slot runs into the next chunk
+        cy.pre = __builtin_amdgcn_readlane(d.pre, WAVE - 1);
+        cy.kv = __builtin_amdgcn_readlane(d.kv, WAVE - 1);
+        cy.base = __builtin_amdgcn_readlane(d.base, WAVE - 1);
+        cy.c = __builtin_amdgcn_readlane(d.c, WAVE - 1);
+        cy.o_lo = __builtin_amdgcn_readlane(d.o_lo, WAVE - 1);
+        cy.o_hi = __builtin_amdgcn_readlane(d.o_hi, WAVE - 1);
+        const uint32_t x = x0 + (uint32_t)lane;
+        const uint32_t r = x - d.pre;
+        const uint32_t om = d.kv & 0xFFFFu;
         uint32_t pos;
-        if (of) {
-            pos = ob + r;
+        if ((d.kv >> 16) & 1u) {
+            pos = d.base + r;
         } else {
             // the (r+1)-th highest set bit of om: largest p with popc(om >> p) >= r + 1
             uint32_t p = 0;
 #pragma unroll
             for (int sh = W / 2; sh >= 1; sh >>= 1)
                 p = ((uint32_t)__popc(om >> (p + sh)) >= r + 1u) ? p + sh : p;
-            pos = ob + p;
+            pos = d.base + p;
         }
-        if (x < total) {
+        if (x < total && r < d.c) {
+            const uint64_t oo = ((uint64_t)d.o_hi << 32) | d.o_lo;
             if (MODE == W_VALUES) out_val[oo + r] = vals[pos];
             else out_idx[oo + r] = pos;
         }
@@ -334,8 +366,10 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
     __shared__ __align__(16) S s_end[WIN];
     __shared__ __align__(16) uint32_t s_br[WIN];
     __shared__ uint32_t s_tile, s_next;
+    __shared__ EmitSlot s_emit[(MODE == W_COUNT) ? 1 : T_THREADS];   // per-wave emission slots
     const int lane = lane_id();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    if (MODE != W_COUNT) s_emit[threadIdx.x].kv = 0u;   // emission slots start free (own wave's)
 
     while (true) {
         // ---- take the next tile (workgroup-level work stealing)
@@ -462,7 +496,8 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
                 }
             }
             if (MODE != W_COUNT) {
-                emit_segments<S, MODE>(c_emit, e_m, e_base, e_asc, e_o, lane, out_idx, vals, out_val);
+                emit_segments<S, MODE>(c_emit, e_m, e_base, e_asc, e_o, lane, s_emit + (threadIdx.x & ~63u),
+                                       out_idx, vals, out_val);
             }
             if (has && cur == NONE) {
                 if (MODE == W_COUNT) {
