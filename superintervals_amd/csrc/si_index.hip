@@ -134,6 +134,32 @@ __global__ __launch_bounds__(256) void k_tree_level(const S* __restrict__ in, si
     if (lane_id() == 0) out[b] = v;
 }
 
+// exitp[i]: where the line walk leaves position i's line [b0, b0 + W) (W = 64 bytes of
+// ends) when every position of [b0, i] is a miss.  The reference walk entering that run of
+// misses at i follows branch[] until it drops below b0; the first such element is the
+// nearest j < b0 with ends[j] >= max(ends[b0..i]), i.e. branch[a] for the leftmost maximum
+// a of ends[b0..i] (all of [b0, a) is below ends[a]).  It depends on i only, not on the
+// query, so the build computes it once and the walk replaces a W-element argmax by one read.
+template <typename S>
+__global__ __launch_bounds__(256) void k_exit(const S* __restrict__ ends, const uint32_t* __restrict__ branch,
+                                              uint32_t n, uint32_t npad, uint32_t* __restrict__ ex) {
+    constexpr uint32_t W = 64 / sizeof(S);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npad) return;
+    if (i >= n) {
+        ex[i] = NONE;
+        return;
+    }
+    const uint32_t b0 = i & ~(W - 1);
+    uint32_t a = b0;
+    S best = ends[b0];
+    for (uint32_t j = b0 + 1; j <= i; ++j) {
+        const S v = ends[j];
+        if (v > best) { best = v; a = j; }
+    }
+    ex[i] = branch[a];
+}
+
 // branch[i] = nearest j < i with ends[j] >= ends[i], or NONE.
 template <typename S>
 __global__ __launch_bounds__(256) void k_branch(TreeDesc<S> td, uint32_t* __restrict__ branch) {
@@ -217,6 +243,7 @@ int build_impl(siIndex* ix, const void* starts, const void* ends, const int32_t*
     HIP_TRY(hipMalloc(&ix->d_starts, npad * sizeof(S)));
     HIP_TRY(hipMalloc(&ix->d_ends, npad * sizeof(S)));
     HIP_TRY(hipMalloc(&ix->d_branch, npad * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&ix->d_exit, npad * sizeof(uint32_t)));
     HIP_TRY(hipMalloc(&ix->d_perm, n * sizeof(uint32_t)));
     if (values) HIP_TRY(hipMalloc(&ix->d_values, npad * sizeof(int32_t)));
     if (npad > n) {
@@ -295,6 +322,9 @@ int build_impl(siIndex* ix, const void* starts, const void* ends, const int32_t*
     }
     ix->n = n;
     k_branch<S><<<g, 256, 0, st>>>(ix->tree<S>(), ix->d_branch);
+    LAUNCH_CHECK();
+    k_exit<S><<<grid_for(npad, 256), 256, 0, st>>>((const S*)ix->d_ends, ix->d_branch, (uint32_t)n, (uint32_t)npad,
+                                                 ix->d_exit);
     LAUNCH_CHECK();
     // ends in ascending order, for the rank form of count
     {

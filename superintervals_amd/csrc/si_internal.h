@@ -19,6 +19,7 @@ struct siIndex {
     void* d_starts = nullptr;
     void* d_ends = nullptr;
     uint32_t* d_branch = nullptr;
+    uint32_t* d_exit = nullptr;      // line exit targets (IndexView::exitp)
     uint32_t* d_perm = nullptr;
     int32_t* d_values = nullptr;
     void* d_tree = nullptr;   // levels 1.. of the max tree, contiguous
@@ -29,10 +30,10 @@ struct siIndex {
     double last_build_ms = 0.0;
 
     void release() {
-        hipFree(d_starts); hipFree(d_ends); hipFree(d_branch); hipFree(d_perm);
+        hipFree(d_starts); hipFree(d_ends); hipFree(d_branch); hipFree(d_exit); hipFree(d_perm);
         hipFree(d_values); hipFree(d_tree); hipFree(d_ends_sorted);
         d_starts = d_ends = d_tree = d_ends_sorted = nullptr;
-        d_branch = d_perm = nullptr;
+        d_branch = d_exit = d_perm = nullptr;
         d_values = nullptr;
         n = 0;
         nlev = 0;
@@ -45,6 +46,7 @@ struct siIndex {
         v.starts = (const S*)d_starts;
         v.ends = (const S*)d_ends;
         v.branch = d_branch;
+        v.exitp = d_exit;
         v.n = (uint32_t)n;
         v.npad = (uint32_t)((n + 63) & ~(size_t)63);
         return v;

@@ -4,6 +4,9 @@
 //   starts[n], ends[n]   S, sorted by (start asc, end desc), padded to a multiple of
 //                        64 entries (starts with the maximum, ends with the minimum)
 //   branch[n]            uint32, nearest j < i with ends[j] >= ends[i], or NONE (padded)
+//   exitp[n]             uint32, the walk's exit from position i's 64-byte line of ends when
+//                        [line start, i] are all misses: branch[leftmost argmax of ends
+//                        over that range] (padded with NONE)
 //   perm[n]              uint32, input position of the interval at sorted position k
 //   values[n]            int32 payload in sorted order (optional)
 //   tree levels          S, a 64-ary max tree over ends: level l+1 holds the max of
@@ -32,6 +35,7 @@ struct IndexView {
     const S* __restrict__ starts;
     const S* __restrict__ ends;
     const uint32_t* __restrict__ branch;
+    const uint32_t* __restrict__ exitp;   // line exit target of each position (see k_exit)
     uint32_t n;
     uint32_t npad;   // allocated length of starts/ends/branch (multiple of 64)
 };

@@ -11,7 +11,7 @@
 //     cache lines; each sorted query gets its upper_bound position.
 //  2. Count: a persistent kernel walks every query.  A workgroup takes a tile of
 //     TILE consecutive sorted queries from a global counter, stages the window of
-//     ends[] and branch[] just below the tile's largest upper_bound in LDS (64 KB),
+//     ends[] and exitp[] just below the tile's largest upper_bound in LDS (64 KB),
 //     and walks from LDS; the few reads below the window go to HBM.  One lane owns
 //     one query; when it finishes, the lane takes the next query from a pool of 64
 //     the wave grabs from the tile's LDS counter, so lanes stay busy however uneven
@@ -26,8 +26,9 @@
 // (just below the lowest hit, or cur when there is none) and follow branch[] from
 // there; that chain's first element below b0 is the nearest j < b0 with
 // ends[j] >= max(ends[b0..x]), which is branch[a] for the leftmost maximum a of
-// ends[b0..x].  One line load and at most one branch load replace the reference's
-// per-element steps while visiting the same hits in the same order.
+// ends[b0..x].  That target depends on x only, so the build stores it per position
+// (exitp[x], see si_index.hip k_exit).  One line load and at most one exitp load replace
+// the reference's per-element steps while visiting the same hits in the same order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -238,30 +239,21 @@ __device__ __forceinline__ void load_line(const S* __restrict__ ends, uint32_t b
     memcpy(L.e, v, 64);
 }
 
-// Bit k set when position b0 + k (k <= kc) is a hit.
+// Bit k set when position b0 + k (k <= kc) is a hit.  The miss bits are shifted in
+// from the top (miss + miss + bit: one compare and one add-with-carry per element).
 template <typename S>
 __device__ __forceinline__ uint32_t hit_mask(const Line<S>& L, uint32_t kc, S s) {
-    uint32_t m = 0;
+    uint32_t miss = 0;
 #pragma unroll
-    for (int k = 0; k < Line<S>::W; ++k) m |= (((uint32_t)k <= kc) & (L.e[k] >= s) ? 1u : 0u) << k;
-    return m;
+    for (int k = Line<S>::W - 1; k >= 0; --k) miss = miss + miss + (uint32_t)(L.e[k] < s);
+    return ~miss & ((2u << kc) - 1u);   // kc < W <= 16
 }
 
-// Offset in the line of the leftmost maximum of ends over the misses [b0, b0 + x], where
-// x is just below the lowest hit (or kc when the line has none).  Only called when b0
-// itself is not a hit; the walk then continues at branch[b0 + offset].
-template <typename S>
-__device__ __forceinline__ uint32_t exit_pos(const Line<S>& L, uint32_t kc, uint32_t m) {
-    const uint32_t x = m ? (uint32_t)(__builtin_ctz(m) - 1) : kc;
-    S best = L.e[0];
-    uint32_t a = 0;
-#pragma unroll
-    for (int k = 1; k < Line<S>::W; ++k) {
-        bool take = ((uint32_t)k <= x) & (L.e[k] > best);
-        best = take ? L.e[k] : best;
-        a = take ? (uint32_t)k : a;
-    }
-    return a;
+// Offset in the line of the top of the trailing misses [b0, b0 + x]: just below the
+// lowest hit, or kc when the line has none.  Only used when b0 itself is not a hit; the
+// walk then continues at exitp[b0 + x] (branch[] of the leftmost maximum of that run).
+__device__ __forceinline__ uint32_t miss_top(uint32_t kc, uint32_t m) {
+    return m ? (uint32_t)(__builtin_ctz(m) - 1) : kc;
 }
 
 // One slot of a wave's emission scratch in LDS: the segment that starts at this output
@@ -343,7 +335,7 @@ __device__ __forceinline__ void emit_segments(uint32_t c, uint32_t m, uint32_t b
     }
 }
 
-// LDS window of a tile: ends and branch of positions [wlo, wlo + WIN), which always
+// LDS window of a tile: ends and exitp of positions [wlo, wlo + WIN), which always
 // ends at or above the tile's largest upper_bound.  64 KB per workgroup.
 template <typename S> struct TileCfg;
 template <> struct TileCfg<int32_t> { static constexpr uint32_t WIN = 8128; };   // 8128 x 8 B
@@ -364,7 +356,7 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
     constexpr int W = Line<S>::W;
     constexpr uint32_t WIN = TileCfg<S>::WIN;
     __shared__ __align__(16) S s_end[WIN];
-    __shared__ __align__(16) uint32_t s_br[WIN];
+    __shared__ __align__(16) uint32_t s_ex[WIN];   // exit targets (IndexView::exitp)
     __shared__ uint32_t s_tile, s_next;
     __shared__ EmitSlot s_emit[(MODE == W_COUNT) ? 1 : T_THREADS];   // per-wave emission slots
     const int lane = lane_id();
@@ -388,9 +380,9 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
         }
         {
             const uint4* ge = reinterpret_cast<const uint4*>(ix.ends + wlo);
-            const uint4* gb = reinterpret_cast<const uint4*>(ix.branch + wlo);
+            const uint4* gb = reinterpret_cast<const uint4*>(ix.exitp + wlo);
             uint4* le = reinterpret_cast<uint4*>(s_end);
-            uint4* lb = reinterpret_cast<uint4*>(s_br);
+            uint4* lb = reinterpret_cast<uint4*>(s_ex);
             const uint32_t ne = wcnt * (uint32_t)sizeof(S) / 16u, nb = wcnt / 4u;
             for (uint32_t j = threadIdx.x; j < ne; j += T_THREADS) le[j] = ge[j];
             for (uint32_t j = threadIdx.x; j < nb; j += T_THREADS) lb[j] = gb[j];
@@ -447,7 +439,7 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
                 const uint32_t kc = cur - b0;
                 const bool in_win = b0 >= wlo;   // every position visited is <= whi
                 Line<S> L;
-                if (in_win) memcpy(L.e, &s_end[b0 - wlo], 64);
+                if (in_win) load_line(s_end, b0 - wlo, L);
                 else load_line(ix.ends, b0, L);
                 const uint32_t m = hit_mask(L, kc, s);
                 if (MODE == W_VECTOR && lead) {
@@ -483,14 +475,13 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
                     if (m & 1u) {
                         cur = b0 - 1u;   // b0 == 0 wraps to NONE
                     } else {
-                        const uint32_t a = exit_pos(L, kc, m);
-                        const uint32_t p = b0 + a;
+                        const uint32_t p = b0 + miss_top(kc, m);
                         // LDS read always (clamped), global read only below the window; the
                         // empty asm pins the LDS value so the two loads are not folded into
                         // one generic (flat) load through a selected pointer
-                        uint32_t nb = s_br[in_win ? p - wlo : 0u];
+                        uint32_t nb = s_ex[in_win ? p - wlo : 0u];
                         asm volatile("" : "+v"(nb));
-                        if (!in_win) nb = ix.branch[p];
+                        if (!in_win) nb = ix.exitp[p];
                         cur = nb;
                     }
                 }
