@@ -265,9 +265,9 @@ int siCoverageHost(const siIndex* ix, const void* qs, const void* qe, size_t nq,
 
 /* Deterministic generators for the benchmark workloads 1-4 (see workload.cpp).
  * starts/ends are host buffers of n coordinates of coord_bytes each. */
-int siWorkloadIntervals(int config, uint64_t seed, size_t n, int coord_bytes, int sorted,
+int siWorkloadIntervals(int workload, uint64_t seed, size_t n, int coord_bytes, int sorted,
                         void* starts, void* ends);
-int siWorkloadQueries(int config, uint64_t seed, size_t n, int coord_bytes, void* qs, void* qe);
+int siWorkloadQueries(int workload, uint64_t seed, size_t n, int coord_bytes, void* qs, void* qe);
 
 #ifdef __cplusplus
 }

@@ -536,7 +536,8 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanV
                                                           int32_t* __restrict__ out_val) {
     const int lane = lane_id();
     const uint64_t le_mask = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);   // lanes <= this
-    const uint32_t nheavy = pv.counter[1];
+    const uint32_t nheavy = __builtin_amdgcn_readfirstlane(pv.counter[1]);
+    if (nheavy == 0) return;   // nothing parked: no wave touches the shared counter
     while (true) {
         uint32_t h = 0;
         if (lane == 0) h = atomicAdd(&pv.counter[2], 1u);

@@ -67,7 +67,7 @@ struct Gen {
         for (int k = 0; k < 10; ++k) { acc += std::pow(100.0, kSegPerm[k] / 9.0) / tot; seg_cdf[k] = acc; }
         seg_cdf[9] = 1.0;
         // Hotspot centres depend on the interval seed's family only, so the
-        // queries of config 3 aim at the same hotspots as the intervals.
+        // queries of workload 3 aim at the same hotspots as the intervals.
         uint64_t hk = stream_key(0x5EED0F407507ULL, ST_HOT);
         for (int k = 0; k < 200; ++k) hot[k] = (int64_t)below(draw(hk, k), 3000000000ULL);
     }
