@@ -27,6 +27,15 @@ if not os.path.exists(LIB_PATH):
         f"{LIB_PATH} is missing: build it first (make -C superintervals_amd/csrc, "
         "or __graft_entry__.build())")
 
+# PyTorch for ROCm ships its own HIP runtime (same soname, libamdhip64.so.7, but loaded by
+# torch under the name libamdhip64.so).  Loading torch first lets this library bind to
+# that runtime; loading this library first would put a second HIP/HSA runtime into the
+# process, and whichever initialises later can find no device.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 lib = C.CDLL(LIB_PATH)
 
 _vp, _sz, _i, _u64p = C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_uint64)

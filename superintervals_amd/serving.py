@@ -43,9 +43,14 @@ class Request:
     arrival: float = 0.0
     done: float = 0.0
     offsets: np.ndarray = None
-    idx: np.ndarray = None
+    idx: np.ndarray = None            # positions in the packed index (int32 view of the result buffer)
+    base: int = 0                     # packed position of the target set's first interval
     counts: np.ndarray = None
     event: threading.Event = field(default_factory=threading.Event)
+
+    def positions(self) -> np.ndarray:
+        """Result positions in the target set alone (uint32), in descending order per query."""
+        return (self.idx.astype(np.int64) - self.base).astype(np.uint32)
 
     @property
     def nq(self):
@@ -93,13 +98,15 @@ class Server:
     background thread started by start()) drives the GPU."""
 
     def __init__(self, resident: ResidentSet, hbm_budget_bytes: int = 8 << 30, streams: int = 3,
-                 max_batch_queries: int = 1 << 22):
+                 max_batch_queries: int = 1 << 22, keep_results: bool = True):
         import torch
+        torch.cuda.init()          # torch's device state before its first stream is created
         self.torch = torch
         self.res = resident
         self.budget = int(hbm_budget_bytes)
         self.in_use = 0
         self.max_batch = int(max_batch_queries)
+        self.keep_results = keep_results       # False: record latency only (load generators)
         self.dev = torch.device("cuda:0")
         self.streams = [torch.cuda.Stream() for _ in range(streams)]
         self.pending = collections.deque()      # waiting for stage 1
@@ -245,9 +252,10 @@ class Server:
             q0 += r.nq
             if r.kind != "search":
                 continue
-            base = self.res.base[r.index_id]
-            r.offsets = (o - o[0]).astype(np.uint64)
-            r.idx = (idx[o[0]:o[-1]].astype(np.int64) - base).astype(np.uint32)
+            r.base = int(self.res.base[r.index_id])
+            if self.keep_results:
+                r.offsets = (o - o[0]).astype(np.uint64)
+                r.idx = idx[o[0]:o[-1]]
             r.done = now
             r.event.set()
             self.completed.append(r)

@@ -44,7 +44,7 @@ def _check(O, oix, reqs, done):
         else:
             off, idx = O.search_batch(oix[i], qs, qe, mode=0)
             assert np.array_equal(r.offsets, off), rid
-            assert np.array_equal(r.idx.astype(np.int64), idx), rid
+            assert np.array_equal(r.positions().astype(np.int64), idx), rid
 
 
 def test_resident_set_positions(setup):
