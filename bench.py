@@ -47,6 +47,8 @@ def parse():
     p.add_argument("--serving-max-n", type=int, default=50_000_000)
     p.add_argument("--serving-rates", type=str, default="50,200,800,3200")
     p.add_argument("--serving-seconds", type=float, default=3.0)
+    p.add_argument("--walk-once", action="store_true",
+                   help="plan the 10M x 10M batch, then one count walk and one fill walk (for PMC counters)")
     return p.parse_args()
 
 
@@ -230,6 +232,33 @@ def bench_serving(a):
     print(json.dumps(out))
 
 
+def walk_once(a):
+    """One count walk and one descending fill walk over the planned 10M x 10M batch."""
+    import numpy as np
+    import torch
+    import superintervals_amd as sia
+    from superintervals_amd import autotune
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    s, e = sia.workload_intervals(2, a.n, seed=1, sorted=True)
+    qs, qe = sia.workload_queries(2, a.nq, seed=2)
+    ix = sia.DeviceIndex(np.int32).build(s, e)
+    nq = a.nq
+    tqs, tqe = torch.from_numpy(qs).to(dev), torch.from_numpy(qe).to(dev)
+    plan = torch.empty(ix.plan_bytes(nq), dtype=torch.uint8, device=dev)
+    off = torch.empty(nq + 1, dtype=torch.int64, device=dev)
+    counts = torch.empty(nq, dtype=torch.int32, device=dev)
+    ix.plan_device(tqs, tqe, plan)
+    ix.offsets_planned(plan, nq, off)
+    torch.cuda.synchronize()
+    idx = torch.empty(int(off[-1].item()), dtype=torch.int32, device=dev)
+    autotune.apply({"count_rank": 0})
+    ix.count_planned(plan, nq, counts)
+    ix.fill_planned(plan, nq, off, idx)
+    torch.cuda.synchronize()
+    print(json.dumps({"walk_once": True, "queries": nq, "hits": int(off[-1].item())}))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -239,6 +268,8 @@ def main():
         return bench_learned(a)
     if a.serving:
         return bench_serving(a)
+    if a.walk_once:
+        return walk_once(a)
     import numpy as np
     import torch
     import superintervals_amd as sia
