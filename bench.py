@@ -47,6 +47,9 @@ def parse():
     p.add_argument("--serving-max-n", type=int, default=50_000_000)
     p.add_argument("--serving-rates", type=str, default="50,200,800,3200")
     p.add_argument("--serving-seconds", type=float, default=3.0)
+    p.add_argument("--graph", action="store_true",
+                   help="small batches: eager plan/offsets/fill vs one captured graph replay")
+    p.add_argument("--graph-nq", type=int, default=4096)
     p.add_argument("--walk-once", action="store_true",
                    help="plan the 10M x 10M batch, then one count walk and one fill walk (for PMC counters)")
     return p.parse_args()
@@ -232,6 +235,78 @@ def bench_serving(a):
     print(json.dumps(out))
 
 
+def bench_graph(a):
+    """Launch-bound small batches on the 10M-interval index: search_idxs of graph_nq queries
+    per batch, eager (plan with and without the digit read-back) against the replay of one
+    captured graph of plan -> offsets -> bounded fill."""
+    import numpy as np
+    import torch
+    import superintervals_amd as sia
+    from superintervals_amd import autotune
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    s, e = sia.workload_intervals(2, a.n, seed=1, sorted=True)
+    ix = sia.DeviceIndex(np.int32).build(s, e)
+    nq = a.graph_nq
+    qs, qe = sia.workload_queries(2, nq, seed=5)
+    tqs, tqe = torch.from_numpy(qs).to(dev), torch.from_numpy(qe).to(dev)
+    plan = torch.empty(ix.plan_bytes(nq), dtype=torch.uint8, device=dev)
+    off = torch.empty(nq + 1, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream()
+    ix.plan_device(tqs, tqe, plan, stream=stream)
+    ix.offsets_planned(plan, nq, off, stream=stream)
+    torch.cuda.synchronize()
+    cap = int(off[-1].item())
+    idx = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+    reps = 200
+    saved = autotune.current()
+
+    def eager():
+        ix.plan_device(tqs, tqe, plan, stream=stream)
+        ix.offsets_planned(plan, nq, off, stream=stream)
+        ix.fill_planned(plan, nq, off, idx, capacity=cap, stream=stream)
+
+    def timed(fn):
+        for _ in range(5):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e6
+
+    res = {}
+    try:
+        autotune.apply({"plan_sync": 1})
+        res["eager_plan_readback_us"] = timed(eager)
+        autotune.apply({"plan_sync": 0})
+        res["eager_no_readback_us"] = timed(eager)
+        ref = idx.clone()
+        g = torch.cuda.CUDAGraph()
+        cs = torch.cuda.Stream()
+        with torch.cuda.graph(g, stream=cs):
+            st = torch.cuda.current_stream()
+            ix.plan_device(tqs, tqe, plan, stream=st)
+            ix.offsets_planned(plan, nq, off, stream=st)
+            ix.fill_planned(plan, nq, off, idx, capacity=cap, stream=st)
+        res["graph_replay_us"] = timed(g.replay)
+        res["graph_output_identical"] = bool(torch.equal(idx, ref))
+        for ls in (64, 16):
+            autotune.apply({"lane_steps": ls})
+            res["eager_no_readback_lane_steps%d_us" % ls] = timed(eager)
+            res["lane_steps%d_output_identical" % ls] = bool(torch.equal(idx, ref))
+    finally:
+        autotune.apply(saved)
+    out = {"metric": "search_idxs latency per small batch: eager vs captured graph (10M idx)",
+           "value": nq / (res["graph_replay_us"] / 1e6), "unit": "queries/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "int32", "data": "synthetic (seeded log-normal generator)",
+           "config": {"workload": "batches of %d queries on 10M int32 intervals, back to back" % nq,
+                      "intervals": a.n, "queries_per_batch": nq, "hits_per_batch": cap, "reps": reps},
+           **res}
+    print(json.dumps(out))
+
+
 def walk_once(a):
     """One count walk and one descending fill walk over the planned 10M x 10M batch."""
     import numpy as np
@@ -270,6 +345,8 @@ def main():
         return bench_serving(a)
     if a.walk_once:
         return walk_once(a)
+    if a.graph:
+        return bench_graph(a)
     import numpy as np
     import torch
     import superintervals_amd as sia

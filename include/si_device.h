@@ -120,6 +120,9 @@ double siIndexLastBuildMs(const siIndex* ix);
  *   "offsets_rank" 1 (default): siSearchOffsets* count with the rank form; 0: the walk
  *   "lane_steps"   line steps a lane spends on one query before the walk is parked for
  *                  the wave-cooperative kernel (1 .. 2^30, default 256)
+ *   "plan_sync"    1 (default): siPlanQueries reads the key-digit histograms back once to
+ *                  skip constant digits (one stream synchronisation); 0: it sorts every
+ *                  digit without a read-back, so the planned calls can be graph-captured
  * superintervals_amd/autotune.py measures candidates and stores the best per workload. */
 int siTuneSet(const char* name, int64_t value);
 int64_t siTuneGet(const char* name);   /* -1 for an unknown name */
@@ -130,7 +133,8 @@ int64_t siTuneGet(const char* name);   /* -1 for an unknown name */
  * neighbouring lanes.  A plan is a caller-owned device buffer of siPlanBytes bytes;
  * it is valid for the index and queries it was made from until the index is rebuilt.
  * Calls that share one plan must be ordered on one stream (they reuse its scratch).
- * siPlanQueries synchronises the stream once (radix pass selection). */
+ * siPlanQueries synchronises the stream once (radix pass selection) unless the tuning
+ * knob "plan_sync" is 0. */
 size_t siPlanBytes(const siIndex* ix, size_t nq);
 
 int siPlanQueries(const siIndex* ix, const void* qs, const void* qe, size_t nq, void* plan,
@@ -150,6 +154,17 @@ int siSearchFillPlanned(const siIndex* ix, void* plan, size_t nq, int order,
 /* Payload fill pass for the planned queries (see siSearchValuesFill). */
 int siSearchValuesFillPlanned(const siIndex* ix, void* plan, size_t nq, const uint64_t* offsets,
                               int32_t* values, void* stream);
+
+/* Bounded fill passes: as above, but only output slots [0, capacity) are written (each
+ * query's segment is cut at capacity).  For callers that size the output before the
+ * counts are known, e.g. a captured graph of plan -> offsets -> fill with fixed buffers
+ * (set the tuning knob "plan_sync" to 0 first): after a replay, offsets[nq] <= capacity
+ * means the output is complete.  No reference counterpart (the reference fills growing
+ * std::vectors, superintervals.hpp:879-905). */
+int siSearchFillPlannedBounded(const siIndex* ix, void* plan, size_t nq, int order,
+                               const uint64_t* offsets, uint32_t* idx, uint64_t capacity, void* stream);
+int siSearchValuesFillPlannedBounded(const siIndex* ix, void* plan, size_t nq, const uint64_t* offsets,
+                                     int32_t* values, uint64_t capacity, void* stream);
 
 /* ---- Batched queries on device buffers ------------------------------------
  * These plan the batch internally (stream-ordered scratch) and then run. */

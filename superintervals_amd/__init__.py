@@ -264,9 +264,15 @@ class DeviceIndex:
         N.check(N.lib.siSearchOffsetsPlanned(self._h, _ptr(plan_t), nq, _ptr(counts_t), _ptr(offsets_t),
                                              _stream_of(stream)), "siSearchOffsetsPlanned")
 
-    def fill_planned(self, plan_t, nq, offsets_t, idx_t, order="desc", stream=None):
-        N.check(N.lib.siSearchFillPlanned(self._h, _ptr(plan_t), nq, _ORDERS[order], _ptr(offsets_t),
-                                          _ptr(idx_t), _stream_of(stream)), "siSearchFillPlanned")
+    def fill_planned(self, plan_t, nq, offsets_t, idx_t, order="desc", stream=None, capacity=None):
+        """Fill pass; with capacity, only output slots [0, capacity) are written."""
+        if capacity is None:
+            N.check(N.lib.siSearchFillPlanned(self._h, _ptr(plan_t), nq, _ORDERS[order], _ptr(offsets_t),
+                                              _ptr(idx_t), _stream_of(stream)), "siSearchFillPlanned")
+        else:
+            N.check(N.lib.siSearchFillPlannedBounded(self._h, _ptr(plan_t), nq, _ORDERS[order], _ptr(offsets_t),
+                                                     _ptr(idx_t), int(capacity), _stream_of(stream)),
+                    "siSearchFillPlannedBounded")
 
     def values_fill_planned(self, plan_t, nq, offsets_t, vals_t, stream=None):
         N.check(N.lib.siSearchValuesFillPlanned(self._h, _ptr(plan_t), nq, _ptr(offsets_t), _ptr(vals_t),

@@ -81,6 +81,8 @@ _SIGS = {
     "siSearchOffsetsPlanned": (_i, [_vp, _vp, _sz, _vp, _vp, _vp]),
     "siSearchFillPlanned": (_i, [_vp, _vp, _sz, _i, _vp, _vp, _vp]),
     "siSearchValuesFillPlanned": (_i, [_vp, _vp, _sz, _vp, _vp, _vp]),
+    "siSearchFillPlannedBounded": (_i, [_vp, _vp, _sz, _i, _vp, _vp, C.c_uint64, _vp]),
+    "siSearchValuesFillPlannedBounded": (_i, [_vp, _vp, _sz, _vp, _vp, C.c_uint64, _vp]),
     "siCountHost": (_i, [_vp, _vp, _vp, _sz, _vp]),
     "siSearchHost": (_i, [_vp, _vp, _vp, _sz, _i, _vp, _vp, _vp, _sz, _vp]),
     "siUpperBoundHost": (_i, [_vp, _vp, _sz, _vp]),

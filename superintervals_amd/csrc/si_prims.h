@@ -329,22 +329,26 @@ __global__ void k_zero_u32(uint32_t* __restrict__ p, size_t n) {
 
 // Stable sort of (keys, vals) by key digits [digit_lo, digit_hi) (8-bit digits,
 // least significant first).  Digits that are equal for every key are skipped, which
-// needs one host read-back of the digit histograms.  tmp comes from rs_temp_carve.  On
-// return *keys_sorted / *vals_sorted point at whichever buffer holds the result.
+// needs one host read-back of the digit histograms; skip_constant = false sorts every
+// digit in [digit_lo, digit_hi) with no read-back, so the sort can be captured in a graph.
+// tmp comes from rs_temp_carve.  On return *keys_sorted / *vals_sorted point at whichever
+// buffer holds the result.
 int radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, size_t n,
                      int digit_lo, int digit_hi, const RsTemp& tmp, uint64_t** keys_sorted,
-                     uint32_t** vals_sorted, hipStream_t st) {
+                     uint32_t** vals_sorted, hipStream_t st, bool skip_constant = true) {
     *keys_sorted = k0;
     *vals_sorted = v0;
     if (n < 2) return SI_OK;
-    k_zero_u32<<<grid_for(8 * RS_BINS, 256), 256, 0, st>>>(tmp.hist8, 8 * RS_BINS);
-    LAUNCH_CHECK();
-    unsigned hb = std::min<unsigned>(grid_for(n, 256 * 16), 2048);
-    k_rs_hist8<<<hb, 256, 0, st>>>(k0, n, tmp.hist8);
-    LAUNCH_CHECK();
-    uint32_t h8[8 * RS_BINS];
-    HIP_TRY(hipMemcpyAsync(h8, tmp.hist8, sizeof(h8), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    uint32_t h8[8 * RS_BINS] = {0};
+    if (skip_constant) {
+        k_zero_u32<<<grid_for(8 * RS_BINS, 256), 256, 0, st>>>(tmp.hist8, 8 * RS_BINS);
+        LAUNCH_CHECK();
+        unsigned hb = std::min<unsigned>(grid_for(n, 256 * 16), 2048);
+        k_rs_hist8<<<hb, 256, 0, st>>>(k0, n, tmp.hist8);
+        LAUNCH_CHECK();
+        HIP_TRY(hipMemcpyAsync(h8, tmp.hist8, sizeof(h8), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
 
     const uint32_t ntiles = (uint32_t)rs_ntiles(n);
     const size_t nth = (size_t)ntiles * RS_BINS;
@@ -352,7 +356,7 @@ int radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, siz
     uint32_t *va = v0, *vb = v1;
     for (int d = digit_lo; d < digit_hi; ++d) {
         bool constant = false;
-        for (int b = 0; b < RS_BINS; ++b)
+        for (int b = 0; skip_constant && b < RS_BINS; ++b)
             if (h8[d * RS_BINS + b] == n) { constant = true; break; }
         if (constant) continue;
         const int shift = 8 * d;

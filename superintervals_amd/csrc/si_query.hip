@@ -193,6 +193,29 @@ __global__ __launch_bounds__(Q_THREADS) void k_count_rank(IndexView<S> ix, const
     if (counts64) counts64[q] = c;
 }
 
+// Queries per tile and resident workgroups of the persistent walk kernel.
+// Launch choices of the walk kernels, set through siTuneSet (the autotuner's knobs).
+//   tile         sorted queries per workgroup tile
+//   count_wgs    resident workgroups per CU for the count pass (0 = occupancy limit)
+//   fill_wgs     resident workgroups per CU for the fill passes (0 = occupancy limit)
+//   count_rank   siCount / siCountPlanned use the rank form instead of the walk
+//   offsets_rank siSearchOffsets* use the rank form (default) instead of the walk
+//   lane_steps   line steps a lane spends on one query before parking it for the
+//                wave-cooperative kernel
+//   plan_sync    1: the plan's radix sort reads the digit histograms back to skip constant
+//                digits (one stream synchronisation); 0: it sorts every key digit without
+//                a host read-back, so plan -> offsets -> fill can be captured in a graph
+struct Tuning {
+    uint32_t tile = 2048;
+    int count_wgs = 0;
+    int fill_wgs = 0;
+    int count_rank = 0;
+    int offsets_rank = 1;
+    uint32_t lane_steps = 256;
+    int plan_sync = 1;
+};
+Tuning g_tune;
+
 template <typename S>
 int plan_impl(const siIndex* ix, const S* qs, const S* qe, size_t nq, void* plan, hipStream_t st) {
     PlanLayout L(sizeof(S), nq);
@@ -209,11 +232,11 @@ int plan_impl(const siIndex* ix, const S* qs, const S* qe, size_t nq, void* plan
     if (sizeof(S) == 4) {
         k_plan_keys_i32<<<g, Q_THREADS, 0, st>>>((const int32_t*)qs, (const int32_t*)qe, nq, k0, v0);
         LAUNCH_CHECK();
-        rc = radix_sort_pairs(k0, v0, k1, v1, nq, 4, 8, rt, &ks, &vs, st);
+        rc = radix_sort_pairs(k0, v0, k1, v1, nq, 4, 8, rt, &ks, &vs, st, g_tune.plan_sync != 0);
     } else {
         k_plan_keys_i64<<<g, Q_THREADS, 0, st>>>((const int64_t*)qe, nq, k0, v0);
         LAUNCH_CHECK();
-        rc = radix_sort_pairs(k0, v0, k1, v1, nq, 0, 8, rt, &ks, &vs, st);
+        rc = radix_sort_pairs(k0, v0, k1, v1, nq, 0, 8, rt, &ks, &vs, st, g_tune.plan_sync != 0);
     }
     if (rc != SI_OK) return rc;
     k_plan_finish<S><<<g, Q_THREADS, 0, st>>>(ix->view<S>(), ks, vs, qs, nq, plan_view<S>(plan, nq));
@@ -352,7 +375,7 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
                                                     const uint64_t* __restrict__ offsets,
                                                     uint32_t* __restrict__ out_idx,
                                                     const int32_t* __restrict__ vals,
-                                                    int32_t* __restrict__ out_val) {
+                                                    int32_t* __restrict__ out_val, uint64_t cap) {
     constexpr int W = Line<S>::W;
     constexpr uint32_t WIN = TileCfg<S>::WIN;
     __shared__ __align__(16) S s_end[WIN];
@@ -417,7 +440,7 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
                     kq = k;
                     steps = 0;
                     has = true;
-                    if (MODE != W_COUNT) { o = offsets[q]; oend = offsets[q + 1]; }
+                    if (MODE != W_COUNT) { o = min(offsets[q], cap); oend = min(offsets[q + 1], cap); }
                     if (MODE == W_VECTOR) { top = cur; lead = true; }
                 }
                 const uint32_t take = min((uint32_t)__popcll(nm), pool_left);
@@ -533,7 +556,7 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanV
                                                           const uint64_t* __restrict__ offsets,
                                                           uint32_t* __restrict__ out_idx,
                                                           const int32_t* __restrict__ vals,
-                                                          int32_t* __restrict__ out_val) {
+                                                          int32_t* __restrict__ out_val, uint64_t cap) {
     const int lane = lane_id();
     const uint64_t le_mask = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);   // lanes <= this
     const uint32_t nheavy = __builtin_amdgcn_readfirstlane(pv.counter[1]);
@@ -551,7 +574,7 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanV
         const uint32_t top = __builtin_amdgcn_readfirstlane(he.top);
         bool lead = __builtin_amdgcn_readfirstlane(he.lead) != 0;
         uint64_t acc = he.acc;
-        const uint64_t oend = (MODE != W_COUNT) ? offsets[q + 1] : 0;
+        const uint64_t oend = (MODE != W_COUNT) ? min(offsets[q + 1], cap) : 0;
         while (cur != NONE) {
             const uint32_t b0 = cur & ~63u;
             const uint32_t kc = cur - b0;
@@ -695,24 +718,6 @@ __global__ __launch_bounds__(Q_THREADS) void k_coverage_i32(IndexView<int32_t> i
     ((ix)->cb == 4 ? [&] { using S = int32_t; return __VA_ARGS__; }()        \
                    : [&] { using S = int64_t; return __VA_ARGS__; }())
 
-// Queries per tile and resident workgroups of the persistent walk kernel.
-// Launch choices of the walk kernels, set through siTuneSet (the autotuner's knobs).
-//   tile         sorted queries per workgroup tile
-//   count_wgs    resident workgroups per CU for the count pass (0 = occupancy limit)
-//   fill_wgs     resident workgroups per CU for the fill passes (0 = occupancy limit)
-//   count_rank   siCount / siCountPlanned use the rank form instead of the walk
-//   offsets_rank siSearchOffsets* use the rank form (default) instead of the walk
-//   lane_steps   line steps a lane spends on one query before parking it for the
-//                wave-cooperative kernel
-struct Tuning {
-    uint32_t tile = 2048;
-    int count_wgs = 0;
-    int fill_wgs = 0;
-    int count_rank = 0;
-    int offsets_rank = 1;
-    uint32_t lane_steps = 256;
-};
-Tuning g_tune;
 
 template <typename S>
 int launch_rank(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint64_t* counts64,
@@ -724,8 +729,12 @@ int launch_rank(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint
     return SI_OK;
 }
 
+// Launch shape of the persistent walk: tile size, number of tiles and resident workgroups.
+// A batch too small to give every resident workgroup a full tile gets smaller tiles (at
+// least one 64-query pool each), so its queries spread over all CUs instead of queueing in
+// the lanes of a few workgroups.
 template <typename S, int MODE>
-unsigned walk_grid(uint32_t ntiles) {
+void walk_shape(size_t nq, uint32_t* tile, uint32_t* ntiles, unsigned* grid) {
     static int cus = 0, per = 0;
     if (cus == 0) {
         int dev = 0;
@@ -737,21 +746,29 @@ unsigned walk_grid(uint32_t ntiles) {
     }
     const int want = (MODE == W_COUNT) ? g_tune.count_wgs : g_tune.fill_wgs;
     const int wgs = (want > 0) ? std::min(want, per) : per;
-    return std::max(1u, std::min((unsigned)(cus * wgs), ntiles));
+    const size_t resident = (size_t)cus * (size_t)wgs;
+    size_t t = g_tune.tile;
+    if (nq < t * resident) t = std::max<size_t>(POOL, (nq + resident - 1) / resident);
+    t = (t + POOL - 1) / POOL * POOL;
+    *tile = (uint32_t)std::min<size_t>(t, g_tune.tile);
+    *ntiles = (uint32_t)((nq + *tile - 1) / *tile);
+    *grid = (unsigned)std::max<size_t>(1, std::min<size_t>(resident, *ntiles));
 }
 
 template <typename S, int MODE>
 int launch_walk(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint64_t* counts64,
-                const uint64_t* offsets, uint32_t* out_idx, int32_t* out_val, hipStream_t st) {
+                const uint64_t* offsets, uint32_t* out_idx, int32_t* out_val, hipStream_t st,
+                uint64_t cap = ~0ull) {
     if (nq == 0) return SI_OK;
     PlanView<S> pv = plan_view<S>(plan, nq);
-    const uint32_t tile = g_tune.tile;
-    const uint32_t ntiles = (uint32_t)((nq + tile - 1) / tile);
+    uint32_t tile = 0, ntiles = 0;
+    unsigned grid = 0;
+    walk_shape<S, MODE>(nq, &tile, &ntiles, &grid);
     k_zero_u32<<<1, 64, 0, st>>>(pv.counter, 3);
     LAUNCH_CHECK();
-    k_walk<S, MODE><<<walk_grid<S, MODE>(ntiles), T_THREADS, 0, st>>>(
+    k_walk<S, MODE><<<grid, T_THREADS, 0, st>>>(
         ix->view<S>(), pv, (uint32_t)nq, tile, ntiles, g_tune.lane_steps, pv.counter, counts, counts64,
-        offsets, out_idx, ix->d_values, out_val);
+        offsets, out_idx, ix->d_values, out_val, cap);
     LAUNCH_CHECK();
     // parked (heavy) walks: one wave each, resident grid, exits at once when none were parked
     static int cus = 0;
@@ -762,7 +779,7 @@ int launch_walk(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint
         cus = std::max(cus, 1);
     }
     k_walk_heavy<S, MODE><<<(unsigned)cus * 8u, Q_THREADS, 0, st>>>(ix->view<S>(), pv, counts, counts64, offsets,
-                                                                   out_idx, ix->d_values, out_val);
+                                                                   out_idx, ix->d_values, out_val, cap);
     LAUNCH_CHECK();
     return SI_OK;
 }
@@ -808,6 +825,9 @@ SI_EXPORT int siTuneSet(const char* name, int64_t value) {
     } else if (k == "lane_steps") {
         if (value < 1 || value > (1 << 30)) return si_set_error(SI_ERR_ARG, "lane_steps must be in [1, 2^30]");
         g_tune.lane_steps = (uint32_t)value;
+    } else if (k == "plan_sync") {
+        if (value != 0 && value != 1) return si_set_error(SI_ERR_ARG, "plan_sync must be 0 or 1");
+        g_tune.plan_sync = (int)value;
     } else if (k == "count_rank" || k == "offsets_rank") {
         if (value != 0 && value != 1) return si_set_error(SI_ERR_ARG, "%s must be 0 or 1", name);
         (k == "count_rank" ? g_tune.count_rank : g_tune.offsets_rank) = (int)value;
@@ -824,6 +844,7 @@ SI_EXPORT int64_t siTuneGet(const char* name) {
     if (k == "count_wgs") return g_tune.count_wgs;
     if (k == "fill_wgs") return g_tune.fill_wgs;
     if (k == "lane_steps") return g_tune.lane_steps;
+    if (k == "plan_sync") return g_tune.plan_sync;
     if (k == "count_rank") return g_tune.count_rank;
     if (k == "offsets_rank") return g_tune.offsets_rank;
     return -1;
@@ -873,26 +894,42 @@ SI_EXPORT int siSearchOffsetsPlanned(const siIndex* ix, void* plan, size_t nq, u
     return device_excl_scan<uint64_t, uint64_t>(offsets, nq, offsets, offsets + nq, ws, st);
 }
 
-SI_EXPORT int siSearchFillPlanned(const siIndex* ix, void* plan, size_t nq, int order,
-                                  const uint64_t* offsets, uint32_t* idx, void* stream) {
+SI_EXPORT int siSearchFillPlannedBounded(const siIndex* ix, void* plan, size_t nq, int order,
+                                         const uint64_t* offsets, uint32_t* idx, uint64_t capacity,
+                                         void* stream) {
     int rc = check_ix(ix);
     if (rc) return rc;
     if (order != SI_ORDER_DESC && order != SI_ORDER_VECTOR) return si_set_error(SI_ERR_ARG, "bad order %d", order);
-    if (nq && (!plan || !offsets || !idx)) return si_set_error(SI_ERR_ARG, "null buffer");
+    if (nq && (!plan || !offsets || (!idx && capacity))) return si_set_error(SI_ERR_ARG, "null buffer");
     hipStream_t st = (hipStream_t)stream;
     if (order == SI_ORDER_DESC)
-        return DISPATCH_S(ix, (launch_walk<S, W_DESC>(ix, plan, nq, nullptr, nullptr, offsets, idx, nullptr, st)));
-    return DISPATCH_S(ix, (launch_walk<S, W_VECTOR>(ix, plan, nq, nullptr, nullptr, offsets, idx, nullptr, st)));
+        return DISPATCH_S(ix, (launch_walk<S, W_DESC>(ix, plan, nq, nullptr, nullptr, offsets, idx, nullptr, st,
+                                                      capacity)));
+    return DISPATCH_S(ix, (launch_walk<S, W_VECTOR>(ix, plan, nq, nullptr, nullptr, offsets, idx, nullptr, st,
+                                                    capacity)));
+}
+
+SI_EXPORT int siSearchFillPlanned(const siIndex* ix, void* plan, size_t nq, int order,
+                                  const uint64_t* offsets, uint32_t* idx, void* stream) {
+    if (nq && !idx) return si_set_error(SI_ERR_ARG, "null buffer");
+    return siSearchFillPlannedBounded(ix, plan, nq, order, offsets, idx, ~0ull, stream);
+}
+
+SI_EXPORT int siSearchValuesFillPlannedBounded(const siIndex* ix, void* plan, size_t nq, const uint64_t* offsets,
+                                               int32_t* values, uint64_t capacity, void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (nq && (!plan || !offsets || (!values && capacity))) return si_set_error(SI_ERR_ARG, "null buffer");
+    if (!ix->d_values) return si_set_error(SI_ERR_ARG, "index was built without values");
+    hipStream_t st = (hipStream_t)stream;
+    return DISPATCH_S(ix, (launch_walk<S, W_VALUES>(ix, plan, nq, nullptr, nullptr, offsets, nullptr, values, st,
+                                                    capacity)));
 }
 
 SI_EXPORT int siSearchValuesFillPlanned(const siIndex* ix, void* plan, size_t nq, const uint64_t* offsets,
                                         int32_t* values, void* stream) {
-    int rc = check_ix(ix);
-    if (rc) return rc;
-    if (nq && (!plan || !offsets || !values)) return si_set_error(SI_ERR_ARG, "null buffer");
-    if (!ix->d_values) return si_set_error(SI_ERR_ARG, "index was built without values");
-    hipStream_t st = (hipStream_t)stream;
-    return DISPATCH_S(ix, (launch_walk<S, W_VALUES>(ix, plan, nq, nullptr, nullptr, offsets, nullptr, values, st)));
+    if (nq && !values) return si_set_error(SI_ERR_ARG, "null buffer");
+    return siSearchValuesFillPlannedBounded(ix, plan, nq, offsets, values, ~0ull, stream);
 }
 
 // ============================================================================
