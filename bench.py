@@ -41,6 +41,7 @@ def parse():
                    help="learned vs plain upper_bound (200M piecewise-density intervals, 50M queries)")
     p.add_argument("--learned-n", type=int, default=200_000_000)
     p.add_argument("--learned-nq", type=int, default=50_000_000)
+    p.add_argument("--learned-steps", type=int, default=400)
     p.add_argument("--serving", action="store_true",
                    help="continuous-batching server: 24 resident indexes, Poisson request stream")
     p.add_argument("--serving-sets", type=int, default=24)
@@ -97,7 +98,7 @@ def bench_learned(a):
     t_plain = time_it(lambda: sia._native.check(lib.siUpperBound(ix.handle, xs.data_ptr(), nq, plain.data_ptr(),
                                                                  None), "siUpperBound"), a.steps)
     res = {}
-    for name, steps in (("untrained", 0), ("trained", 400)):
+    for name, steps in (("untrained", 0), ("trained", a.learned_steps)):
         m = sia.LearnedIndex(ix, 4096)
         ev = events(2)
         ev[0].record(stream)
@@ -106,14 +107,15 @@ def bench_learned(a):
         torch.cuda.synchronize()
         t_train = ev[0].elapsed_time(ev[1])
         out = torch.empty(nq, dtype=torch.int32, device=dev)
-        st = torch.zeros(2, dtype=torch.int64, device=dev)
+        st = torch.zeros(3, dtype=torch.int64, device=dev)
         m.upper_bound_device(xs, out, st, stream=stream)
         torch.cuda.synchronize()
-        fb, w = st.cpu().tolist()
+        fb, w, corr = st.cpu().tolist()
         same = bool(torch.equal(out, plain))
         t = time_it(lambda: m.upper_bound_device(xs, out, None, stream=stream), a.steps)
         res[name] = {"ms": t, "queries_per_s": nq / (t / 1e3), "identical_to_plain": same,
-                     "fallbacks": fb, "mean_window": w / nq, "train_ms": t_train, **info}
+                     "fallbacks": fb, "mean_window": w / nq, "mean_leaf_correction": corr / nq,
+                     "train_ms": t_train, **info}
         print("[learned] %s: %.3f ms (plain %.3f ms), identical %s, max error %d" %
               (name, t, t_plain, same, info["max_abs_err"]), file=sys.stderr, flush=True)
         m.close()
@@ -124,7 +126,7 @@ def bench_learned(a):
         "higher_is_better": True, "dtype": "int32 keys, fp32 model",
         "data": "synthetic (seeded piecewise-density generator, inputs resident in HBM)",
         "config": {"workload": "upper_bound of 50M query ends over 200M int32 starts (10 density segments)",
-                   "intervals": n, "queries": nq, "leaves": 4096, "train_steps": 400, "batch": 65536,
+                   "intervals": n, "queries": nq, "leaves": 4096, "train_steps": a.learned_steps, "batch": 65536,
                    "host_generation_s": round(gen_s, 2), "build_ms": round(ix.last_build_ms, 2)},
         "plain": {"ms": t_plain, "queries_per_s": nq / (t_plain / 1e3)},
         "learned": res,

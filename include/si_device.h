@@ -220,14 +220,16 @@ int siWalkStats(const siIndex* ix, const void* qs, const void* qe, size_t nq, ui
                 void* stream);
 
 /* ---- Learned position index (no counterpart in the reference) ---------------
- * A two-stage FP32 model of rank(start): a root MLP 1 -> 16 -> 16 -> 1 that picks one of
- * `leaves` linear models, each with the exact min / max error over the keys it serves.
- * siUpperBoundLearned searches only the predicted window, checks the result against the
- * neighbours outside it and falls back to the plain binary search when the check fails,
- * so its output always equals siUpperBound's. */
+ * A two-stage FP32 model of rank(start): a root MLP 1 -> 16 -> 16 -> 1 predicts which of
+ * `leaves` linear models serves a key; leaf l owns the positions [l n / L, (l + 1) n / L)
+ * and carries the exact bounds of its prediction error over every key it serves.
+ * siUpperBoundLearned corrects the predicted leaf by a search of the leaves' first keys,
+ * searches only that leaf's window, and checks the result (falling back to the whole
+ * leaf), so its output always equals siUpperBound's. */
 typedef struct siLearned siLearned;
 
-/* Attach an untrained model to a built index (the index must outlive the model). */
+/* Attach an untrained model to a built index (the index must outlive the model).
+ * leaves in [1, 8192] (at most the index size). */
 siLearned* siLearnedCreate(const siIndex* ix, uint32_t leaves);
 void siLearnedDestroy(siLearned* m);
 
@@ -244,8 +246,8 @@ int siLearnedGetParams(const siLearned* m, float* params);
 /* Normalisation of keys: x = (key - kmin) * inv_span. */
 int siLearnedNorm(const siLearned* m, int64_t* kmin, float* inv_span);
 
-/* max_abs_err: largest |position error| of any leaf over its keys; last_loss: final
- * minibatch loss; used_leaves: leaves serving at least one key.  Each may be NULL. */
+/* max_abs_err: largest |error bound| of any leaf; last_loss: final minibatch loss;
+ * used_leaves: number of leaves.  Each may be NULL. */
 int siLearnedInfo(const siLearned* m, int64_t* max_abs_err, float* last_loss, uint32_t* used_leaves);
 
 /* Training-loss gradient at params (host, NULL = the model's) over the given sample
@@ -254,7 +256,8 @@ int siLearnedInfo(const siLearned* m, int64_t* max_abs_err, float* last_loss, ui
 int siLearnedGradient(siLearned* m, const float* params, const uint32_t* pos, uint32_t count, float* grad);
 
 /* upper_bound through the model, device buffers; identical to siUpperBound.  stats (device,
- * may be NULL) accumulates [0] fallbacks to the plain search and [1] total window width. */
+ * may be NULL, 3 entries) accumulates [0] fallbacks to the whole-leaf search, [1] total
+ * window width, [2] total |served leaf - predicted leaf|. */
 int siUpperBoundLearned(const siLearned* m, const void* xs, size_t n, uint32_t* out, uint64_t* stats,
                         void* stream);
 

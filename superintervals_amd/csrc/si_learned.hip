@@ -7,18 +7,19 @@
 // Model (FP32), trained from the index's own (start -> rank) pairs:
 //   root   x = (key - key_min) / (key_max - key_min);  h1 = relu(W1 x + b1) (16),
 //          h2 = relu(W2 h1 + b2) (16),  y = w3 . h2 + b3  ~ rank / n;
-//          leaf = clamp(floor(y * L), 0, L - 1) with L = 4096 leaves.
+//          predicted leaf = clamp(floor(y * L), 0, L - 1) with L = 4096 leaves.
 //          Initialised as the identity CDF (h1_j = relu(x - j/16), W2 = I, y = h2_0),
-//          i.e. the untrained network routes by x alone.
-//   leaf   p = base + c + a * (float)(key - key0), in leaf-local integer offsets so
-//          FP32 keeps full precision; a, c fitted by least squares over the keys the
-//          root routes to the leaf; [elo, ehi] the exact min / max of
-//          (true upper_bound - p) over those keys.
+//          i.e. the untrained network predicts the leaf from x alone.
+//   leaf   leaf l owns positions [l n / L, (l + 1) n / L); p = base + c + a (key - key0)
+//          in leaf-local integer offsets so FP32 keeps full precision, a and c fitted by
+//          least squares over the leaf's keys; [elo, ehi] the exact bounds of
+//          (upper_bound - p) over every key the leaf serves.
 // Training: minibatch MSE on (x_i, i / n) with hand-written forward + backward kernels
 // (per-block partial gradients, then a fixed-order reduction: deterministic) and Adam.
-// Query: p, then a binary search of starts in [p + elo - 1, p + ehi] (clamped), then a
-// check that the result is the global upper_bound (the entries just outside the window
-// bracket the key); if not, the plain binary search answers.
+// Query: the root's leaf, corrected by a galloping search of the leaves' first keys
+// (LDS) to the leaf that holds upper_bound(x); a binary search of starts in the leaf's
+// window; a one-read check of the result, with a whole-leaf search as the fallback.  A
+// better root means fewer correction steps; the output never depends on it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,8 +40,9 @@ constexpr int RMI_THREADS = 256;
 constexpr int GRAD_SLOTS = NPARAM + 1;         // gradient + loss
 
 struct Leaf {
-    int64_t key0;   // smallest key routed to the leaf (INT64_MAX: empty)
-    int64_t base;   // smallest position routed to the leaf
+    int64_t key0;   // the leaf's first key
+    uint32_t base;  // the leaf's first position
+    uint32_t last;  // the leaf's last position
     float a, c;     // p = base + c + a * (key - key0)
     int32_t elo, ehi;
 };
@@ -75,7 +77,7 @@ __device__ __forceinline__ float norm_key(S key, int64_t kmin, float inv_span) {
 __device__ __forceinline__ int64_t leaf_pred(const Leaf& lf, int64_t key) {
     const float d = (float)(key - lf.key0);
     const float f = fminf(fmaxf(floorf(fmaf(lf.a, d, lf.c)), -1e15f), 1e15f);   // keep the cast defined
-    return lf.base + (int64_t)f;
+    return (int64_t)lf.base + (int64_t)f;
 }
 
 // Training samples: sample j of step t is position hash(t, j) mod n.
@@ -203,160 +205,207 @@ __global__ void k_rmi_adam(float* __restrict__ P, float* __restrict__ m, float* 
     P[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
 }
 
-// Leaf statistics, pass 1: smallest key and position routed to each leaf.  Keys are sorted,
-// so the first key routed to a leaf is its smallest.
-template <typename S>
-__global__ __launch_bounds__(RMI_THREADS) void k_leaf_first(const S* __restrict__ starts, uint32_t n,
-                                                            int64_t kmin, float inv_span,
-                                                            const float* __restrict__ P, uint32_t L,
-                                                            uint32_t* __restrict__ leaf_of_key,
-                                                            unsigned long long* __restrict__ first_pos) {
-    __shared__ float s_p[NPARAM];
-    for (int k = threadIdx.x; k < NPARAM; k += RMI_THREADS) s_p[k] = P[k];
-    __syncthreads();
-    const uint32_t i = blockIdx.x * RMI_THREADS + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t l = leaf_of(root_forward(s_p, norm_key(starts[i], kmin, inv_span)), L);
-    leaf_of_key[i] = l;
-    atomicMin(&first_pos[l], (unsigned long long)i);
+// Leaves own contiguous position ranges: leaf l holds positions [P(l), P(l + 1)) with
+// P(l) = l * n / L, so a leaf never mixes keys from separate parts of the index, whatever
+// the root predicts.  The root only has to name the leaf; the query corrects that guess
+// by a galloping search of the leaves' first keys (kept in LDS).
+__host__ __device__ __forceinline__ uint32_t leaf_begin(uint32_t l, uint32_t n, uint32_t L) {
+    return (uint32_t)(((uint64_t)l * n) / L);
 }
 
-// Pass 2: least-squares sums over (d = key - key0, r = pos - base) per leaf, in double.
+// The leaf holding position i: the largest l with P(l) <= i.
+__device__ __forceinline__ uint32_t leaf_of_pos(uint32_t i, uint32_t n, uint32_t L) {
+    return (uint32_t)((((uint64_t)i + 1) * L - 1) / n);
+}
+
+// The leaf's prediction, clamped to its positions.  Monotone non-decreasing in the key
+// (slope >= 0, float rounding, fma, floor and clamp are all monotone), which the error
+// bounds below rely on.
+__device__ __forceinline__ int64_t leaf_pred_c(const Leaf& lf, int64_t key, int64_t lo, int64_t hi) {
+    return std::min<int64_t>(std::max<int64_t>(leaf_pred(lf, key), lo), hi);
+}
+
+// Pass 1: least squares of (position - P(l)) on (key - key0) over the leaf's positions, in
+// double; one block per leaf, so no atomics.  kb[l] receives the leaf's first key.
 template <typename S>
-__global__ __launch_bounds__(RMI_THREADS) void k_leaf_sums(const S* __restrict__ starts, uint32_t n,
-                                                           const uint32_t* __restrict__ leaf_of_key,
-                                                           const unsigned long long* __restrict__ first_pos,
-                                                           double* __restrict__ sums /*[L][5]*/) {
-    const uint32_t i = blockIdx.x * RMI_THREADS + threadIdx.x;
-    const bool valid = i < n;
-    const uint32_t l = valid ? leaf_of_key[i] : 0xFFFFFFFFu;
+__global__ __launch_bounds__(RMI_THREADS) void k_leaf_fit(const S* __restrict__ starts, uint32_t n, uint32_t L,
+                                                          Leaf* __restrict__ leaves, S* __restrict__ kb) {
+    __shared__ double s_v[5][RMI_THREADS / WAVE];
+    const uint32_t l = blockIdx.x;
+    const uint32_t b = leaf_begin(l, n, L), e = leaf_begin(l + 1, n, L);
+    const int64_t key0 = (int64_t)starts[b];
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (valid) {
-        const uint32_t f = (uint32_t)first_pos[l];
-        const double d = (double)((int64_t)starts[i] - (int64_t)starts[f]);
-        const double r = (double)(i - f);
-        v[0] = 1.0; v[1] = d; v[2] = r; v[3] = d * d; v[4] = d * r;
+    for (uint32_t i = b + threadIdx.x; i < e; i += RMI_THREADS) {
+        const double d = (double)((int64_t)starts[i] - key0), r = (double)(i - b);
+        v[0] += 1.0; v[1] += d; v[2] += r; v[3] += d * d; v[4] += d * r;
     }
-    // aggregate the wave when all its valid lanes share one leaf (keys are sorted, so
-    // leaves are mostly runs); lane 0 is valid whenever any lane is
-    const uint32_t l0 = __shfl(l, 0, WAVE);
-    const bool uni = __all(!valid || l == l0);
-    if (uni) {
+    const int w = threadIdx.x / WAVE;
 #pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        double x = v[c];
+#pragma unroll
+        for (int sh = 32; sh >= 1; sh >>= 1) x += __shfl_xor(x, sh, WAVE);
+        if (lane_id() == 0) s_v[c][w] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[5];
         for (int c = 0; c < 5; ++c) {
-            double x = v[c];
-#pragma unroll
-            for (int s = 32; s >= 1; s >>= 1) x += __shfl_xor(x, s, WAVE);
-            v[c] = x;
+            t[c] = 0.0;
+            for (int k = 0; k < RMI_THREADS / WAVE; ++k) t[c] += s_v[c][k];
         }
-        if (lane_id() == 0 && l0 != 0xFFFFFFFFu)
-            for (int c = 0; c < 5; ++c) atomicAdd(&sums[(size_t)l0 * 5 + c], v[c]);
-    } else if (valid) {
-        for (int c = 0; c < 5; ++c) atomicAdd(&sums[(size_t)l * 5 + c], v[c]);
-    }
-}
-
-template <typename S>
-__global__ void k_leaf_fit(const S* __restrict__ starts, uint32_t L, const unsigned long long* __restrict__ first_pos,
-                           const double* __restrict__ sums, Leaf* __restrict__ leaves) {
-    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= L) return;
-    Leaf lf;
-    const unsigned long long f = first_pos[l];
-    const double cnt = sums[(size_t)l * 5 + 0];
-    if (f == ~0ull || cnt <= 0.0) {
-        lf.key0 = INT64_MAX; lf.base = 0; lf.a = 0.f; lf.c = 0.f; lf.elo = 1; lf.ehi = 0;   // empty
-    } else {
-        const double sd = sums[l * 5 + 1], sr = sums[l * 5 + 2], sdd = sums[l * 5 + 3], sdr = sums[l * 5 + 4];
-        const double md = sd / cnt, mr = sr / cnt;
-        const double var = sdd / cnt - md * md, cov = sdr / cnt - md * mr;
-        const double a = (var > 0.0) ? cov / var : 0.0;
-        lf.key0 = (int64_t)starts[f];
-        lf.base = (int64_t)f;
+        const double cnt = t[0];
+        const double md = t[1] / cnt, mr = t[2] / cnt;
+        const double var = t[3] / cnt - md * md, cov = t[4] / cnt - md * mr;
+        const double a = (var > 0.0 && cov > 0.0) ? cov / var : 0.0;
+        Leaf lf;
+        lf.key0 = key0;
+        lf.base = b;
+        lf.last = e - 1;
         lf.a = (float)a;
         lf.c = (float)(mr - a * md);
         lf.elo = INT32_MAX;
         lf.ehi = INT32_MIN;
+        leaves[l] = lf;
+        kb[l] = (S)key0;
     }
-    leaves[l] = lf;
 }
 
-// Pass 3: exact error bounds.  Only the last key of a run of equal starts is a distinct
-// upper_bound value (ub(key) = that position).
+// Pass 2: exact bounds [elo, ehi] of upper_bound(x) - pred(x) over every key x the leaf
+// serves (kb[l] <= x < kb[l + 1]).  upper_bound is constant between consecutive distinct
+// keys and pred is monotone, so the extremes sit at the ends of those pieces: at each key's
+// last position (ub = that position), just below each key's first position (ub = first - 1,
+// pred <= pred(key)), and just below the next leaf's first key (ub = the leaf's last
+// position; for the last leaf, keys above every start clamp to it as well).
 template <typename S>
-__global__ __launch_bounds__(RMI_THREADS) void k_leaf_err(const S* __restrict__ starts, uint32_t n,
-                                                          const uint32_t* __restrict__ leaf_of_key,
+__global__ __launch_bounds__(RMI_THREADS) void k_leaf_err(const S* __restrict__ starts, uint32_t n, uint32_t L,
                                                           Leaf* __restrict__ leaves) {
     const uint32_t i = blockIdx.x * RMI_THREADS + threadIdx.x;
     if (i >= n) return;
-    if (i + 1 < n && starts[i + 1] == starts[i]) return;
-    const uint32_t l = leaf_of_key[i];
+    const uint32_t l = leaf_of_pos(i, n, L);
+    const uint32_t pl = leaf_begin(l, n, L), ph = leaf_begin(l + 1, n, L) - 1;
     const Leaf lf = leaves[l];
-    int64_t e = (int64_t)i - leaf_pred(lf, (int64_t)starts[i]);
-    e = std::max<int64_t>(std::min<int64_t>(e, INT32_MAX / 2), INT32_MIN / 2);
-    atomicMin(&leaves[l].elo, (int32_t)e);
-    atomicMax(&leaves[l].ehi, (int32_t)e);
+    const S k = starts[i];
+    const int64_t p = leaf_pred_c(lf, (int64_t)k, pl, ph);
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    if (i + 1 == n || starts[i + 1] != k) lo = hi = (int64_t)i - p;
+    if (i > pl && starts[i - 1] != k) {
+        const int64_t e1 = (int64_t)i - 1 - p;
+        lo = std::min(lo, e1);
+        hi = std::max(hi, e1);
+    }
+    if (i == ph) {   // the top piece of the leaf: keys up to the next leaf's first key
+        const int64_t ptop = (i + 1 < n) ? leaf_pred_c(lf, (int64_t)starts[i + 1] - 1, pl, ph) : (int64_t)ph;
+        const int64_t e2 = (int64_t)ph - ptop;
+        lo = std::min(lo, e2);
+        hi = std::max(hi, e2);
+    }
+    if (lo > hi) return;
+    lo = std::max<int64_t>(lo, INT32_MIN / 2);
+    hi = std::min<int64_t>(hi, INT32_MAX / 2);
+    atomicMin(&leaves[l].elo, (int32_t)lo);
+    atomicMax(&leaves[l].ehi, (int32_t)hi);
 }
 
-// Learned upper_bound.  stats (optional) counts fallbacks and sums window widths.
+// Last r in [lo, hi] with starts[r] <= x, or lo - 1 (branchless halving).
+template <typename S>
+__device__ __forceinline__ int64_t last_le(const S* __restrict__ st, int64_t lo, int64_t hi, S x) {
+    const S* base = st + lo;
+    uint32_t len = (uint32_t)(hi - lo + 1);
+    while (len > 1) {
+        const uint32_t half = len >> 1;
+        base = (base[half] <= x) ? base + half : base;
+        len -= half;
+    }
+    return (int64_t)(base - st) - ((*base <= x) ? 0 : 1);
+}
+
+// Learned upper_bound.  The root names a leaf; a galloping search of the leaves' first
+// keys in LDS corrects it to the leaf holding upper_bound(x); the leaf's window (which
+// provably contains the answer) is searched; a one-read check guards the result and falls
+// back to a search of the whole leaf.  Persistent blocks stage the root and the first keys
+// once.  stats (optional): [0] fallbacks, [1] window widths, [2] leaf corrections.
 template <typename S>
 __global__ __launch_bounds__(RMI_THREADS) void k_ub_learned(IndexView<S> ix, int64_t kmin, float inv_span,
                                                             const float* __restrict__ P,
-                                                            const Leaf* __restrict__ leaves, uint32_t L,
+                                                            const Leaf* __restrict__ leaves,
+                                                            const S* __restrict__ kb, uint32_t L,
                                                             const S* __restrict__ xs, size_t nx,
                                                             uint32_t* __restrict__ out,
                                                             unsigned long long* __restrict__ stats) {
-    __shared__ float s_p[NPARAM];
+    extern __shared__ __align__(16) char s_mem[];
+    float* s_p = reinterpret_cast<float*>(s_mem);
+    S* s_kb = reinterpret_cast<S*>(s_mem + ((NPARAM * 4 + 15) & ~15));
     for (int k = threadIdx.x; k < NPARAM; k += RMI_THREADS) s_p[k] = P[k];
+    for (uint32_t k = threadIdx.x; k < L; k += RMI_THREADS) s_kb[k] = kb[k];
     __syncthreads();
-    const size_t q = (size_t)blockIdx.x * RMI_THREADS + threadIdx.x;
-    const bool valid = q < nx;
-    const S x = valid ? xs[q] : (S)0;
-    const uint32_t n = ix.n;
-    uint32_t res = NONE;
-    bool ok = false;
-    uint32_t width = 0;
-    if (!valid || n == 0) {
-        ok = true;
-    } else {
-        const Leaf lf = leaves[leaf_of(root_forward(s_p, norm_key(x, kmin, inv_span)), L)];
-        if (lf.elo <= lf.ehi) {
-            const int64_t p = leaf_pred(lf, (int64_t)x);
-            int64_t lo = p + lf.elo - 1, hi = p + lf.ehi;
-            lo = std::max<int64_t>(lo, 0);
-            hi = std::min<int64_t>(hi, (int64_t)n - 1);
-            if (lo <= hi) {
-                width = (uint32_t)(hi - lo + 1);
-                // last r in [lo, hi] with starts[r] <= x, or lo - 1
-                const S* base = ix.starts + lo;
-                uint32_t len = (uint32_t)(hi - lo + 1);
-                while (len > 1) {
-                    const uint32_t half = len >> 1;
-                    base = (base[half] <= x) ? base + half : base;
-                    len -= half;
-                }
-                const int64_t r = (int64_t)(base - ix.starts) - ((*base <= x) ? 0 : 1);
-                const bool top_ok = (r < hi) || (hi == (int64_t)n - 1) || (ix.starts[hi + 1] > x);
-                const bool bot_ok = (r >= lo) || (lo == 0) || (ix.starts[lo - 1] <= x);
-                if (top_ok && bot_ok) {
-                    ok = true;
-                    res = (r < 0) ? NONE : (uint32_t)r;
-                }
+    unsigned long long f_acc = 0, w_acc = 0, c_acc = 0;
+    for (size_t q0 = (size_t)blockIdx.x * RMI_THREADS; q0 < nx; q0 += (size_t)gridDim.x * RMI_THREADS) {
+        const size_t q = q0 + threadIdx.x;
+        if (q >= nx) continue;
+        const S x = xs[q];
+        const int32_t lg = (int32_t)leaf_of(root_forward(s_p, norm_key(x, kmin, inv_span)), L);
+        // bracket the leaf: s_kb[lo] <= x < s_kb[hi], lo in [-1, L - 1], hi in [lo + 1, L]
+        int32_t lo, hi;
+        if (s_kb[lg] <= x) {
+            lo = lg;
+            hi = lg + 1;
+            int32_t step = 1;
+            while (hi < (int32_t)L && s_kb[hi] <= x) {
+                lo = hi;
+                step <<= 1;
+                hi = lo + step;
             }
+            hi = std::min(hi, (int32_t)L);
+        } else {
+            hi = lg;
+            lo = lg - 1;
+            int32_t step = 1;
+            while (lo >= 0 && s_kb[lo] > x) {
+                hi = lo;
+                step <<= 1;
+                lo = hi - step;
+            }
+            lo = std::max(lo, -1);
         }
+        while (hi - lo > 1) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (s_kb[mid] <= x) lo = mid;
+            else hi = mid;
+        }
+        uint32_t res = NONE;
+        if (lo >= 0) {
+            c_acc += (unsigned long long)(lo > lg ? lo - lg : lg - lo);
+            const Leaf lf = leaves[lo];
+            const int64_t pl = lf.base, ph = lf.last;
+            const int64_t p = leaf_pred_c(lf, (int64_t)x, pl, ph);
+            const int64_t wl = std::max<int64_t>(p + lf.elo, pl), wh = std::min<int64_t>(p + lf.ehi, ph);
+            bool ok = false;
+            int64_t r = pl - 1;
+            if (wl <= wh) {
+                w_acc += (unsigned long long)(wh - wl + 1);
+                r = last_le(ix.starts, wl, wh, x);
+                ok = (r >= wl) && (r < wh || wh == ph || ix.starts[wh + 1] > x);
+            }
+            if (!ok) {
+                f_acc += 1;
+                r = last_le(ix.starts, pl, ph, x);   // starts[pl] <= x, so r >= pl
+            }
+            res = (uint32_t)r;
+        }
+        out[q] = res;
     }
-    if (!ok) res = upper_bound_pos(ix.starts, n, x);
-    if (valid) out[q] = res;
     if (stats) {
-        unsigned long long f = ok ? 0ull : 1ull, w = width;
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) {
-            f += __shfl_xor(f, d, WAVE);
-            w += __shfl_xor(w, d, WAVE);
+            f_acc += __shfl_xor(f_acc, d, WAVE);
+            w_acc += __shfl_xor(w_acc, d, WAVE);
+            c_acc += __shfl_xor(c_acc, d, WAVE);
         }
         if (lane_id() == 0) {
-            atomicAdd(&stats[0], f);
-            atomicAdd(&stats[1], w);
+            atomicAdd(&stats[0], f_acc);
+            atomicAdd(&stats[1], w_acc);
+            atomicAdd(&stats[2], c_acc);
         }
     }
 }
@@ -372,6 +421,7 @@ struct siLearned {
     float inv_span = 1.f;
     float* d_params = nullptr;   // NPARAM
     Leaf* d_leaves = nullptr;    // L
+    void* d_kb = nullptr;        // L first keys of the leaves (coordinate type)
     bool trained = false;
     int steps_done = 0;
     float last_loss = 0.f;
@@ -382,6 +432,7 @@ struct siLearned {
     ~siLearned() {
         (void)hipFree(d_params);
         (void)hipFree(d_leaves);
+        (void)hipFree(d_kb);
     }
 };
 
@@ -401,22 +452,9 @@ template <typename S>
 int fit_leaves(siLearned* m, const siIndex* ix, hipStream_t st) {
     const uint32_t n = (uint32_t)ix->n, L = m->L;
     IndexView<S> v = ix->view<S>();
-    uint32_t* d_leaf = nullptr;
-    unsigned long long* d_first = nullptr;
-    double* d_sums = nullptr;
-    HIP_TRY(hipMalloc(&d_leaf, (size_t)n * 4));
-    HIP_TRY(hipMalloc(&d_first, (size_t)L * 8));
-    HIP_TRY(hipMalloc(&d_sums, (size_t)L * 5 * 8));
-    HIP_TRY(hipMemsetAsync(d_first, 0xFF, (size_t)L * 8, st));
-    HIP_TRY(hipMemsetAsync(d_sums, 0, (size_t)L * 5 * 8, st));
-    const unsigned g = grid_for(n, RMI_THREADS);
-    k_leaf_first<S><<<g, RMI_THREADS, 0, st>>>(v.starts, n, m->kmin, m->inv_span, m->d_params, L, d_leaf, d_first);
+    k_leaf_fit<S><<<L, RMI_THREADS, 0, st>>>(v.starts, n, L, m->d_leaves, (S*)m->d_kb);
     LAUNCH_CHECK();
-    k_leaf_sums<S><<<g, RMI_THREADS, 0, st>>>(v.starts, n, d_leaf, d_first, d_sums);
-    LAUNCH_CHECK();
-    k_leaf_fit<S><<<grid_for(L, 256), 256, 0, st>>>(v.starts, L, d_first, d_sums, m->d_leaves);
-    LAUNCH_CHECK();
-    k_leaf_err<S><<<g, RMI_THREADS, 0, st>>>(v.starts, n, d_leaf, m->d_leaves);
+    k_leaf_err<S><<<grid_for(n, RMI_THREADS), RMI_THREADS, 0, st>>>(v.starts, n, L, m->d_leaves);
     LAUNCH_CHECK();
     HIP_TRY(hipStreamSynchronize(st));
     std::vector<Leaf> h(L);
@@ -425,9 +463,6 @@ int fit_leaves(siLearned* m, const siIndex* ix, hipStream_t st) {
     for (const Leaf& lf : h)
         if (lf.elo <= lf.ehi) mx = std::max(mx, std::max(std::abs(lf.elo), std::abs(lf.ehi)));
     m->max_abs_err = mx;
-    hipFree(d_leaf);
-    hipFree(d_first);
-    hipFree(d_sums);
     return SI_OK;
 }
 
@@ -487,13 +522,14 @@ int train_impl(siLearned* m, const siIndex* ix, int steps, uint32_t batch, float
 SI_EXPORT siLearned* siLearnedCreate(const siIndex* ix, uint32_t leaves) {
     if (check_ix(ix) != SI_OK) return nullptr;
     if (ix->n < 2) { si_set_error(SI_ERR_ARG, "a learned index needs at least 2 intervals"); return nullptr; }
-    if (leaves < 1 || leaves > (1u << 24)) { si_set_error(SI_ERR_ARG, "leaves must be in [1, 2^24]"); return nullptr; }
+    if (leaves < 1 || leaves > 8192) { si_set_error(SI_ERR_ARG, "leaves must be in [1, 8192]"); return nullptr; }
     siLearned* m = new siLearned();
-    m->L = leaves;
+    m->L = (uint32_t)std::min<size_t>(leaves, ix->n);   // every leaf holds at least one position
     m->ix = ix;
     m->ix_n = ix->n;
     if (hipMalloc(&m->d_params, NPARAM * 4) != hipSuccess ||
-        hipMalloc(&m->d_leaves, (size_t)leaves * sizeof(Leaf)) != hipSuccess) {
+        hipMalloc(&m->d_leaves, (size_t)m->L * sizeof(Leaf)) != hipSuccess ||
+        hipMalloc(&m->d_kb, (size_t)m->L * ix->cb) != hipSuccess) {
         si_set_error(SI_ERR_NOMEM, "learned index allocation");
         delete m;
         return nullptr;
@@ -587,7 +623,8 @@ SI_EXPORT int siLearnedNorm(const siLearned* m, int64_t* kmin, float* inv_span) 
 }
 
 // upper_bound through the model (device buffers).  stats (device, may be NULL) receives
-// [0] += fallbacks to the plain search, [1] += sum of window widths.
+// [0] += fallbacks to the whole-leaf search, [1] += sum of window widths, [2] += sum of
+// |leaf - root's leaf| corrections.
 SI_EXPORT int siUpperBoundLearned(const siLearned* m, const void* xs, size_t n, uint32_t* out,
                                   uint64_t* stats, void* stream) {
     if (!m || !m->trained) return si_set_error(SI_ERR_ARG, "model not trained");
@@ -598,15 +635,23 @@ SI_EXPORT int siUpperBoundLearned(const siLearned* m, const void* xs, size_t n, 
     if (n == 0) return SI_OK;
     if (!xs || !out) return si_set_error(SI_ERR_ARG, "null buffer");
     hipStream_t st = (hipStream_t)stream;
-    const unsigned g = grid_for(n, RMI_THREADS);
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        cus = std::max(cus, 1);
+    }
+    const unsigned g = (unsigned)std::min<size_t>(grid_for(n, RMI_THREADS), (size_t)cus * 8);
+    const size_t lds = ((NPARAM * 4 + 15) & ~15) + (size_t)m->L * ix->cb;
     if (ix->cb == 4)
-        k_ub_learned<int32_t><<<g, RMI_THREADS, 0, st>>>(ix->view<int32_t>(), m->kmin, m->inv_span, m->d_params,
-                                                         m->d_leaves, m->L, (const int32_t*)xs, n, out,
-                                                         (unsigned long long*)stats);
+        k_ub_learned<int32_t><<<g, RMI_THREADS, lds, st>>>(ix->view<int32_t>(), m->kmin, m->inv_span, m->d_params,
+                                                           m->d_leaves, (const int32_t*)m->d_kb, m->L,
+                                                           (const int32_t*)xs, n, out, (unsigned long long*)stats);
     else
-        k_ub_learned<int64_t><<<g, RMI_THREADS, 0, st>>>(ix->view<int64_t>(), m->kmin, m->inv_span, m->d_params,
-                                                         m->d_leaves, m->L, (const int64_t*)xs, n, out,
-                                                         (unsigned long long*)stats);
+        k_ub_learned<int64_t><<<g, RMI_THREADS, lds, st>>>(ix->view<int64_t>(), m->kmin, m->inv_span, m->d_params,
+                                                           m->d_leaves, (const int64_t*)m->d_kb, m->L,
+                                                           (const int64_t*)xs, n, out, (unsigned long long*)stats);
     LAUNCH_CHECK();
     return SI_OK;
 }

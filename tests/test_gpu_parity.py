@@ -237,7 +237,7 @@ def test_workload3_wide_queries(sia, orc):
 def test_learned_index(sia, orc):
     """Learned upper_bound (workload 4 distributions, 2M intervals): GPU gradients against the
     float64 oracle, output identical to the plain search, and the trained root's gain over the
-    untrained (identity-CDF) one."""
+    untrained (identity-CDF) one in leaf corrections."""
     import torch
     from oracle import rmi
     print("[gpu] learned index", flush=True)
@@ -277,22 +277,24 @@ def test_learned_index(sia, orc):
     got_plain = plain.cpu().numpy().view(np.uint32).astype(np.int64)
     got_plain[got_plain == 0xFFFFFFFF] = -1
     assert np.array_equal(got_plain, want)
-    widths = {}
+    corr = {}
     for name, m in (("untrained", untrained), ("trained", model)):
         out = torch.empty(xs.size, dtype=torch.int32, device=dev)
-        st = torch.zeros(2, dtype=torch.int64, device=dev)
+        st = torch.zeros(3, dtype=torch.int64, device=dev)
         m.upper_bound_device(xt, out, st)
         got = out.cpu().numpy().view(np.uint32).astype(np.int64)
         got[got == 0xFFFFFFFF] = -1
         assert np.array_equal(got, want), name
-        fb, w = st.cpu().tolist()
-        widths[name] = w / xs.size
-        print(f"[gpu] {name}: fallbacks {fb}, mean window {w / xs.size:.1f}", flush=True)
-    # what the trained network gains: narrower exact windows on average than routing by key
-    # alone (the search cost).  The worst single leaf may widen as the root moves keys
-    # between leaves, so max_abs_err is reported, not compared.
-    print(f"[gpu] max leaf error untrained {info0['max_abs_err']}, trained {info1['max_abs_err']}", flush=True)
-    assert widths["trained"] < widths["untrained"]
+        fb, w, c = st.cpu().tolist()
+        corr[name] = c / xs.size
+        print(f"[gpu] {name}: fallbacks {fb}, mean window {w / xs.size:.1f}, mean leaf correction "
+              f"{c / xs.size:.2f}", flush=True)
+    # Leaves own fixed position ranges, so the windows do not depend on the root; what the
+    # trained network gains is naming the right leaf: fewer correction steps than the
+    # identity-CDF root on a density that varies 100x between segments.
+    print(f"[gpu] max leaf error bound {info1['max_abs_err']}", flush=True)
+    assert info0["max_abs_err"] == info1["max_abs_err"]
+    assert corr["trained"] < 0.25 * corr["untrained"]
     model.close()
     untrained.close()
     ix.close()
