@@ -41,7 +41,7 @@ def parse():
                    help="learned vs plain upper_bound (200M piecewise-density intervals, 50M queries)")
     p.add_argument("--learned-n", type=int, default=200_000_000)
     p.add_argument("--learned-nq", type=int, default=50_000_000)
-    p.add_argument("--learned-steps", type=int, default=400)
+    p.add_argument("--learned-steps", type=int, default=3000)
     p.add_argument("--serving", action="store_true",
                    help="continuous-batching server: 24 resident indexes, Poisson request stream")
     p.add_argument("--serving-sets", type=int, default=24)
