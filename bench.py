@@ -94,9 +94,16 @@ def bench_learned(a):
         torch.cuda.synchronize()
         return ev[0].elapsed_time(ev[1]) / reps
 
+    from superintervals_amd import autotune
     plain = torch.empty(nq, dtype=torch.int32, device=dev)
-    t_plain = time_it(lambda: sia._native.check(lib.siUpperBound(ix.handle, xs.data_ptr(), nq, plain.data_ptr(),
-                                                                 None), "siUpperBound"), a.steps)
+    ub = lambda: sia._native.check(lib.siUpperBound(ix.handle, xs.data_ptr(), nq, plain.data_ptr(), None),
+                                   "siUpperBound")
+    autotune.apply({"ub_tree": 0})
+    t_plain = time_it(ub, a.steps)           # plain halving over all starts
+    ref = plain.clone()
+    autotune.apply({"ub_tree": 1})
+    t_tree = time_it(ub, a.steps)            # Eytzinger top in LDS + block separators
+    tree_same = bool(torch.equal(plain, ref))
     res = {}
     for name, steps in (("untrained", 0), ("trained", a.learned_steps)):
         m = sia.LearnedIndex(ix, 4096)
@@ -111,13 +118,13 @@ def bench_learned(a):
         m.upper_bound_device(xs, out, st, stream=stream)
         torch.cuda.synchronize()
         fb, w, corr = st.cpu().tolist()
-        same = bool(torch.equal(out, plain))
+        same = bool(torch.equal(out, ref))
         t = time_it(lambda: m.upper_bound_device(xs, out, None, stream=stream), a.steps)
         res[name] = {"ms": t, "queries_per_s": nq / (t / 1e3), "identical_to_plain": same,
                      "fallbacks": fb, "mean_window": w / nq, "mean_leaf_correction": corr / nq,
                      "train_ms": t_train, **info}
-        print("[learned] %s: %.3f ms (plain %.3f ms), identical %s, max error %d" %
-              (name, t, t_plain, same, info["max_abs_err"]), file=sys.stderr, flush=True)
+        print("[learned] %s: %.3f ms (plain %.3f ms, tree %.3f ms), identical %s, max error %d" %
+              (name, t, t_plain, t_tree, same, info["max_abs_err"]), file=sys.stderr, flush=True)
         m.close()
     tr = res["trained"]
     out = {
@@ -129,8 +136,10 @@ def bench_learned(a):
                    "intervals": n, "queries": nq, "leaves": 4096, "train_steps": a.learned_steps, "batch": 65536,
                    "host_generation_s": round(gen_s, 2), "build_ms": round(ix.last_build_ms, 2)},
         "plain": {"ms": t_plain, "queries_per_s": nq / (t_plain / 1e3)},
+        "tree": {"ms": t_tree, "queries_per_s": nq / (t_tree / 1e3), "identical_to_plain": tree_same},
         "learned": res,
         "speedup_over_plain": t_plain / tr["ms"],
+        "speedup_over_tree": t_tree / tr["ms"],
         "max_abs_err": tr["max_abs_err"],
     }
     print(json.dumps(out))

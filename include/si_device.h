@@ -120,6 +120,9 @@ double siIndexLastBuildMs(const siIndex* ix);
  *   "offsets_rank" 1 (default): siSearchOffsets* count with the rank form; 0: the walk
  *   "lane_steps"   line steps a lane spends on one query before the walk is parked for
  *                  the wave-cooperative kernel (1 .. 2^30, default 256)
+ *   "ub_tree"      1 (default): siUpperBound descends a 64-ary tree of first keys (top
+ *                  level of at most 4096 keys in Eytzinger order in LDS, then one 64-key
+ *                  node per level down to the starts); 0: plain halving over all starts
  *   "plan_sync"    1 (default): siPlanQueries reads the key-digit histograms back once to
  *                  skip constant digits (one stream synchronisation); 0: it sorts every
  *                  digit without a read-back, so the planned calls can be graph-captured
@@ -212,6 +215,14 @@ int siSearchValuesFill(const siIndex* ix, const void* qs, const void* qe, size_t
  * c_superintervals.h:758-792. */
 int siCoverage(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint32_t* count_out,
                int32_t* cov_out, void* stream);
+
+/* Keys (and payload) of result positions, for batched search_keys / search_items: after a
+ * descending fill (SI_ORDER_DESC), out_starts[k] = starts[idx[k]], out_ends[k] =
+ * ends[idx[k]], out_values[k] = values[idx[k]] (coordinate type for starts/ends; any output
+ * may be NULL).  Together they give superintervals.hpp:913-971 (search_keys, search_items:
+ * the same descending walk) for a whole batch. */
+int siGatherKeys(const siIndex* ix, const uint32_t* idx, size_t n, void* out_starts, void* out_ends,
+                 int32_t* out_values, void* stream);
 
 /* Reference-walk statistics summed over the batch: totals[0] = hits, totals[1] = reads of
  * ends[] by the walk, totals[2] = reads of branch[] (jumps), totals[3] = queries whose

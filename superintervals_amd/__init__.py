@@ -213,6 +213,38 @@ class DeviceIndex:
                                        total, C.byref(tot)), "siSearchHost")
         return off, out
 
+    def search_keys(self, qs, qe, items=False):
+        """Batched search_keys (items=False) or search_items (items=True), superintervals.hpp
+        :913-971: offsets[nq+1] and, per hit in descending walk order, its start and end
+        (and payload).  Runs on the device: plan, offsets, descending fill, key gather."""
+        import torch
+        qs, qe = self._q(qs, qe)
+        nq = qs.size
+        dev = torch.device("cuda:0")
+        tqs, tqe = torch.from_numpy(qs).to(dev), torch.from_numpy(qe).to(dev)
+        off = torch.empty(nq + 1, dtype=torch.int64, device=dev)
+        plan = torch.empty(max(self.plan_bytes(nq), 256), dtype=torch.uint8, device=dev)
+        if nq:
+            self.plan_device(tqs, tqe, plan)
+            self.offsets_planned(plan, nq, off)
+        else:
+            off.zero_()
+        total = int(off[-1].item())
+        tdt = torch.int32 if self.dtype == np.int32 else torch.int64
+        idx = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+        ks = torch.empty(max(total, 1), dtype=tdt, device=dev)
+        ke = torch.empty(max(total, 1), dtype=tdt, device=dev)
+        kv = torch.empty(max(total, 1), dtype=torch.int32, device=dev) if items else None
+        if total:
+            self.fill_planned(plan, nq, off, idx, order="desc")
+            N.check(N.lib.siGatherKeys(self._h, _ptr(idx), total, _ptr(ks), _ptr(ke), _ptr(kv), None),
+                    "siGatherKeys")
+        torch.cuda.synchronize()
+        out = [off.cpu().numpy().astype(np.uint64), ks[:total].cpu().numpy(), ke[:total].cpu().numpy()]
+        if items:
+            out.append(kv[:total].cpu().numpy())
+        return tuple(out)
+
     def walk_stats_device(self, qs_t, qe_t, stream=None):
         """(hits, ends reads, branch reads, found) totals of the reference walk."""
         import torch

@@ -64,6 +64,7 @@ _SIGS = {
     "siSearchValuesFill": (_i, [_vp, _vp, _vp, _sz, _vp, _vp, _vp]),
     "siCoverage": (_i, [_vp, _vp, _vp, _sz, _vp, _vp, _vp]),
     "siWalkStats": (_i, [_vp, _vp, _vp, _sz, _vp, _vp]),
+    "siGatherKeys": (_i, [_vp, _vp, _sz, _vp, _vp, _vp, _vp]),
     "siLearnedCreate": (_vp, [_vp, C.c_uint32]),
     "siLearnedDestroy": (None, [_vp]),
     "siLearnedTrain": (_i, [_vp, _i, C.c_uint32, C.c_float, _vp]),

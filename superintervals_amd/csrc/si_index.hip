@@ -134,6 +134,26 @@ __global__ __launch_bounds__(256) void k_tree_level(const S* __restrict__ in, si
     if (lane_id() == 0) out[b] = v;
 }
 
+// Search tree for upper_bound.  Level l + 1 holds every 64th key of level l (out[j] =
+// in[64 j]), so the 64 keys of level l below entry j of level l + 1 are one 256-byte
+// (int32) node whose first key is that entry.
+template <typename S>
+__global__ __launch_bounds__(256) void k_every64(const S* __restrict__ in, uint32_t n_out, S* __restrict__ out) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n_out) out[j] = in[(size_t)j * 64];
+}
+
+// The top level in Eytzinger order: node k (1-based, k < 2^h) at depth d holds in-order
+// element i = ((2 (k - 2^d) + 1) << (h - 1 - d)) - 1 of the level, or max(S) past its end.
+template <typename S>
+__global__ __launch_bounds__(256) void k_eytz(const S* __restrict__ lvl, uint32_t m, uint32_t h, S* __restrict__ top) {
+    const uint32_t k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (1u << h)) return;
+    const uint32_t d = 31u - (uint32_t)__builtin_clz(k);
+    const uint32_t i = ((2u * (k - (1u << d)) + 1u) << (h - 1u - d)) - 1u;
+    top[k - 1] = (i < m) ? lvl[i] : ((sizeof(S) == 4) ? (S)INT32_MAX : (S)INT64_MAX);
+}
+
 // exitp[i]: where the line walk leaves position i's line [b0, b0 + W) (W = 64 bytes of
 // ends) when every position of [b0, i] is a miss.  The reference walk entering that run of
 // misses at i follows branch[] until it drops below b0; the first such element is the
@@ -326,6 +346,36 @@ int build_impl(siIndex* ix, const void* starts, const void* ends, const int32_t*
     k_exit<S><<<grid_for(npad, 256), 256, 0, st>>>((const S*)ix->d_ends, ix->d_branch, (uint32_t)n, (uint32_t)npad,
                                                  ix->d_exit);
     LAUNCH_CHECK();
+    // upper_bound search tree: 64-ary levels of first keys, top level (<= 4096) in Eytzinger order
+    {
+        uint32_t sz[MAX_LEVELS + 1];
+        sz[0] = (uint32_t)n;
+        int nl = 0;
+        size_t tot = 0;
+        while (sz[nl] > 4095 && nl < MAX_LEVELS - 1) {
+            sz[nl + 1] = (uint32_t)((sz[nl] + 63) / 64);
+            ++nl;
+            ix->lv_off[nl] = tot;
+            ix->lv_size[nl] = sz[nl];
+            tot += sz[nl];
+        }
+        ix->nlv = nl;
+        if (tot) HIP_TRY(hipMalloc(&ix->d_lv, tot * sizeof(S)));
+        for (int l = 1; l <= nl; ++l) {
+            const S* in = (l == 1) ? (const S*)ix->d_starts : (const S*)ix->d_lv + ix->lv_off[l - 1];
+            k_every64<S><<<grid_for(sz[l], 256), 256, 0, st>>>(in, sz[l], (S*)ix->d_lv + ix->lv_off[l]);
+            LAUNCH_CHECK();
+        }
+        const uint32_t m = sz[nl];
+        uint32_t h = 0;
+        while (((1u << h) - 1u) < m) ++h;
+        ix->top_size = m;
+        ix->top_h = h;
+        HIP_TRY(hipMalloc(&ix->d_top, (size_t)((1u << h) - 1u) * sizeof(S)));
+        const S* topl = (nl == 0) ? (const S*)ix->d_starts : (const S*)ix->d_lv + ix->lv_off[nl];
+        k_eytz<S><<<grid_for((1u << h) - 1u, 256), 256, 0, st>>>(topl, m, h, (S*)ix->d_top);
+        LAUNCH_CHECK();
+    }
     // ends in ascending order, for the rank form of count
     {
         const size_t kb = align256(n * 8), vb = align256(n * 4);

@@ -24,6 +24,15 @@ struct siIndex {
     int32_t* d_values = nullptr;
     void* d_tree = nullptr;   // levels 1.. of the max tree, contiguous
     void* d_ends_sorted = nullptr;   // ends in ascending order (rank count)
+    // upper_bound search tree: level l + 1 holds every 64th key of level l (level 0 =
+    // starts) up to a top level of at most 4096 keys, which is also kept in Eytzinger
+    // (breadth-first) order with max(S) padding to 2^top_h - 1 keys
+    void* d_lv = nullptr;            // levels 1 .. nlv, contiguous
+    void* d_top = nullptr;
+    int nlv = 0;                     // levels above the starts
+    size_t lv_off[MAX_LEVELS] = {0}; // element offsets of levels 1 .. nlv in d_lv
+    uint32_t lv_size[MAX_LEVELS] = {0};
+    uint32_t top_size = 0, top_h = 0;
     int nlev = 0;
     size_t lvl_off[MAX_LEVELS] = {0};   // element offsets into d_tree (level >= 1)
     uint32_t lvl_size[MAX_LEVELS] = {0};
@@ -31,8 +40,10 @@ struct siIndex {
 
     void release() {
         hipFree(d_starts); hipFree(d_ends); hipFree(d_branch); hipFree(d_exit); hipFree(d_perm);
-        hipFree(d_values); hipFree(d_tree); hipFree(d_ends_sorted);
-        d_starts = d_ends = d_tree = d_ends_sorted = nullptr;
+        hipFree(d_values); hipFree(d_tree); hipFree(d_ends_sorted); hipFree(d_lv); hipFree(d_top);
+        d_starts = d_ends = d_tree = d_ends_sorted = d_lv = d_top = nullptr;
+        nlv = 0;
+        top_size = top_h = 0;
         d_branch = d_exit = d_perm = nullptr;
         d_values = nullptr;
         n = 0;

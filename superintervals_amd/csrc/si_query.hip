@@ -202,6 +202,8 @@ __global__ __launch_bounds__(Q_THREADS) void k_count_rank(IndexView<S> ix, const
 //   offsets_rank siSearchOffsets* use the rank form (default) instead of the walk
 //   lane_steps   line steps a lane spends on one query before parking it for the
 //                wave-cooperative kernel
+//   ub_tree      1: siUpperBound descends a 64-ary tree of first keys (its top level in
+//                Eytzinger order in LDS); 0: plain halving over all starts
 //   plan_sync    1: the plan's radix sort reads the digit histograms back to skip constant
 //                digits (one stream synchronisation); 0: it sorts every key digit without
 //                a host read-back, so plan -> offsets -> fill can be captured in a graph
@@ -213,6 +215,7 @@ struct Tuning {
     int offsets_rank = 1;
     uint32_t lane_steps = 256;
     int plan_sync = 1;
+    int ub_tree = 1;
 };
 Tuning g_tune;
 
@@ -646,6 +649,61 @@ __global__ __launch_bounds__(Q_THREADS) void k_upper_bound(IndexView<S> ix, cons
     if (q < n) out[q] = upper_bound_pos(ix.starts, ix.n, xs[q]);
 }
 
+// Last r in [lo, hi] with v[r] <= x, or lo - 1 (branchless halving).
+template <typename S>
+__device__ __forceinline__ int64_t last_le_in(const S* __restrict__ v, int64_t lo, int64_t hi, S x) {
+    const S* base = v + lo;
+    uint32_t len = (uint32_t)(hi - lo + 1);
+    while (len > 1) {
+        const uint32_t half = len >> 1;
+        base = (base[half] <= x) ? base + half : base;
+        len -= half;
+    }
+    return (int64_t)(base - v) - ((*base <= x) ? 0 : 1);
+}
+
+// Levels of the upper_bound search tree (level 0 = starts), passed by value.
+template <typename S>
+struct UbTree {
+    const S* lv[MAX_LEVELS];     // lv[0] = starts
+    uint32_t size[MAX_LEVELS];
+    int nlv;                     // levels above the starts
+};
+
+// upper_bound through the search tree: descend the Eytzinger top (LDS) to r = the number of
+// top-level keys <= x, then at each level below search the one 64-key node under the entry
+// found (its first key is that entry, so it is <= x and the node holds the answer for the
+// level).  Persistent blocks stage the top once.  Same result as upper_bound_pos
+// (superintervals.hpp:501-516).
+template <typename S>
+__global__ __launch_bounds__(Q_THREADS) void k_upper_bound_tree(UbTree<S> t, const S* __restrict__ top, uint32_t h,
+                                                                const S* __restrict__ xs, size_t nx,
+                                                                uint32_t* __restrict__ out) {
+    extern __shared__ __align__(16) char s_mem[];
+    S* s_top = reinterpret_cast<S*>(s_mem);
+    const uint32_t T = (1u << h) - 1u;
+    for (uint32_t k = threadIdx.x; k < T; k += Q_THREADS) s_top[k] = top[k];
+    __syncthreads();
+    for (size_t q0 = (size_t)blockIdx.x * Q_THREADS; q0 < nx; q0 += (size_t)gridDim.x * Q_THREADS) {
+        const size_t q = q0 + threadIdx.x;
+        if (q >= nx) continue;
+        const S x = xs[q];
+        uint32_t k = 1, r = 0;
+        for (uint32_t lvl = 0; lvl < h; ++lvl) {
+            const bool right = s_top[k - 1] <= x;
+            r += right ? (1u << (h - 1u - lvl)) : 0u;   // the left subtree and the node itself
+            k = 2u * k + (right ? 1u : 0u);
+        }
+        // padding keys are max(S); a key equal to max(S) may count padding, so clamp
+        int64_t j = (int64_t)std::min<uint32_t>(r, t.size[t.nlv]) - 1;
+        for (int l = t.nlv - 1; l >= 0 && j >= 0; --l) {
+            const int64_t b = j * 64, e = std::min<int64_t>(b + 63, (int64_t)t.size[l] - 1);
+            j = last_le_in(t.lv[l], b, e, x);
+        }
+        out[q] = (j < 0) ? NONE : (uint32_t)j;
+    }
+}
+
 template <typename S>
 __global__ __launch_bounds__(Q_THREADS) void k_has_overlaps(IndexView<S> ix, const S* __restrict__ qs,
                                                             const S* __restrict__ qe, size_t nq,
@@ -682,6 +740,22 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_stats(IndexView<S> ix, const
     }
     __syncthreads();
     if (threadIdx.x < 4 && s_t[threadIdx.x]) atomicAdd(&totals[threadIdx.x], s_t[threadIdx.x]);
+}
+
+// Keys (and payload) of result positions: out_s[k] = starts[idx[k]], out_e[k] =
+// ends[idx[k]], out_v[k] = values[idx[k]] (each output may be NULL).
+template <typename S>
+__global__ __launch_bounds__(Q_THREADS) void k_gather_keys(IndexView<S> ix, const int32_t* __restrict__ vals,
+                                                           const uint32_t* __restrict__ idx, size_t n,
+                                                           S* __restrict__ out_s, S* __restrict__ out_e,
+                                                           int32_t* __restrict__ out_v) {
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = idx[k];
+    if (i >= ix.n) return;   // not a position of this index: left as is
+    if (out_s) out_s[k] = ix.starts[i];
+    if (out_e) out_e[k] = ix.ends[i];
+    if (out_v) out_v[k] = vals[i];
 }
 
 __global__ __launch_bounds__(Q_THREADS) void k_coverage_i32(IndexView<int32_t> ix,
@@ -825,6 +899,9 @@ SI_EXPORT int siTuneSet(const char* name, int64_t value) {
     } else if (k == "lane_steps") {
         if (value < 1 || value > (1 << 30)) return si_set_error(SI_ERR_ARG, "lane_steps must be in [1, 2^30]");
         g_tune.lane_steps = (uint32_t)value;
+    } else if (k == "ub_tree") {
+        if (value != 0 && value != 1) return si_set_error(SI_ERR_ARG, "ub_tree must be 0 or 1");
+        g_tune.ub_tree = (int)value;
     } else if (k == "plan_sync") {
         if (value != 0 && value != 1) return si_set_error(SI_ERR_ARG, "plan_sync must be 0 or 1");
         g_tune.plan_sync = (int)value;
@@ -845,6 +922,7 @@ SI_EXPORT int64_t siTuneGet(const char* name) {
     if (k == "fill_wgs") return g_tune.fill_wgs;
     if (k == "lane_steps") return g_tune.lane_steps;
     if (k == "plan_sync") return g_tune.plan_sync;
+    if (k == "ub_tree") return g_tune.ub_tree;
     if (k == "count_rank") return g_tune.count_rank;
     if (k == "offsets_rank") return g_tune.offsets_rank;
     return -1;
@@ -1009,6 +1087,30 @@ SI_EXPORT int siUpperBound(const siIndex* ix, const void* xs, size_t n, uint32_t
     if (n == 0) return SI_OK;
     if (!xs || !out) return si_set_error(SI_ERR_ARG, "null buffer");
     hipStream_t st = (hipStream_t)stream;
+    if (g_tune.ub_tree && ix->d_top && ix->n > 0) {
+        static int cus = 0;
+        if (cus == 0) {
+            int dev = 0;
+            hipGetDevice(&dev);
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            cus = std::max(cus, 1);
+        }
+        const unsigned g = (unsigned)std::min<size_t>(grid_for(n, Q_THREADS), (size_t)cus * 8);
+        const size_t lds = (size_t)((1u << ix->top_h) - 1u) * ix->cb;
+        return DISPATCH_S(ix, ([&]() -> int {
+            UbTree<S> t;
+            t.nlv = ix->nlv;
+            t.lv[0] = (const S*)ix->d_starts;
+            t.size[0] = (uint32_t)ix->n;
+            for (int l = 1; l < MAX_LEVELS; ++l) {
+                t.lv[l] = (l <= ix->nlv) ? (const S*)ix->d_lv + ix->lv_off[l] : nullptr;
+                t.size[l] = (l <= ix->nlv) ? ix->lv_size[l] : 0u;
+            }
+            k_upper_bound_tree<S><<<g, Q_THREADS, lds, st>>>(t, (const S*)ix->d_top, ix->top_h, (const S*)xs, n, out);
+            LAUNCH_CHECK();
+            return SI_OK;
+        }()));
+    }
     return DISPATCH_S(ix, ([&]() -> int {
         k_upper_bound<S><<<grid_for(n, Q_THREADS), Q_THREADS, 0, st>>>(ix->view<S>(), (const S*)xs, n, out);
         LAUNCH_CHECK();
@@ -1026,6 +1128,22 @@ SI_EXPORT int siHasOverlaps(const siIndex* ix, const void* qs, const void* qe, s
     return DISPATCH_S(ix, ([&]() -> int {
         k_has_overlaps<S><<<grid_for(nq, Q_THREADS), Q_THREADS, 0, st>>>(ix->view<S>(), (const S*)qs,
                                                                         (const S*)qe, nq, out);
+        LAUNCH_CHECK();
+        return SI_OK;
+    }()));
+}
+
+SI_EXPORT int siGatherKeys(const siIndex* ix, const uint32_t* idx, size_t n, void* out_starts, void* out_ends,
+                           int32_t* out_values, void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (n == 0) return SI_OK;
+    if (!idx) return si_set_error(SI_ERR_ARG, "null positions");
+    if (out_values && !ix->d_values) return si_set_error(SI_ERR_ARG, "index was built without values");
+    hipStream_t st = (hipStream_t)stream;
+    return DISPATCH_S(ix, ([&]() -> int {
+        k_gather_keys<S><<<grid_for(n, Q_THREADS), Q_THREADS, 0, st>>>(ix->view<S>(), ix->d_values, idx, n,
+                                                                      (S*)out_starts, (S*)out_ends, out_values);
         LAUNCH_CHECK();
         return SI_OK;
     }()));

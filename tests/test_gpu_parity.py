@@ -130,6 +130,75 @@ def test_random_vs_oracle(sia, orc, seed, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_upper_bound_tree_and_halving(sia, dtype):
+    """Both upper_bound forms (Eytzinger top + block separators, and plain halving) against
+    numpy on sizes around the 64-start blocks and the 4095-key top, long duplicate runs
+    across block boundaries, and keys at the type's extremes."""
+    from superintervals_amd import autotune
+    info = np.iinfo(dtype)
+    rng = np.random.default_rng(91)
+    saved = autotune.current()
+    try:
+        for n in (1, 63, 64, 65, 4095 * 64, 4096 * 64 + 1, 300_001):
+            for kind in range(3):
+                if kind == 0:
+                    s = np.sort(rng.integers(info.min, info.max, n, dtype=np.int64)).astype(dtype)
+                elif kind == 1:
+                    s = np.sort(rng.integers(0, max(2, n // 50), n)).astype(dtype)
+                else:
+                    s = np.sort(np.concatenate([rng.integers(0, 10, n // 2),
+                                                np.full(n - n // 2, info.max, np.int64)])).astype(dtype)
+                ix = _dev_index(sia, s, s)
+                st = ix.download()["starts"]
+                xs = np.concatenate([st[rng.integers(0, n, 500)].astype(np.int64),
+                                     st[rng.integers(0, n, 200)].astype(np.int64) - 1,
+                                     st[rng.integers(0, n, 200)].astype(np.int64) + 1,
+                                     [info.min, info.max, int(st[0]) - 1],
+                                     rng.integers(info.min, info.max, 300, dtype=np.int64)])
+                xs = np.clip(xs, info.min, info.max).astype(dtype)
+                want = np.searchsorted(st, xs, side="right").astype(np.int64) - 1
+                for tree in (1, 0):
+                    autotune.apply({"ub_tree": tree})
+                    got = ix.upper_bound(xs).astype(np.int64)
+                    got[got == 0xFFFFFFFF] = -1
+                    assert np.array_equal(got, want), (n, kind, tree)
+                ix.close()
+    finally:
+        autotune.apply(saved)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_search_keys_and_items(sia, orc, dtype):
+    """Batched search_keys / search_items (superintervals.hpp:913-971): the descending walk's
+    hits as (start, end) and (start, end, value), against the oracle's descending lists."""
+    print(f"[gpu] search_keys/items dtype={np.dtype(dtype).name}", flush=True)
+    rng = np.random.default_rng(77)
+    n = 30_000
+    s = rng.integers(0, 2_000_000, n).astype(dtype)
+    e = (s + rng.geometric(1 / 400, n) - 1).astype(dtype)
+    if dtype == np.int64:
+        s, e = s + (dtype(5) << 36), e + (dtype(5) << 36)
+    vals = rng.integers(-2**31, 2**31 - 1, n).astype(np.int32)
+    qs = rng.integers(0, 2_000_000, 20_000).astype(dtype)
+    qe = (qs + rng.integers(0, 3_000, qs.size)).astype(dtype)
+    if dtype == np.int64:
+        qs, qe = qs + (dtype(5) << 36), qe + (dtype(5) << 36)
+    ix = _dev_index(sia, s, e, vals)
+    oix = orc.build_index(s, e, vals)
+    ooff, oidx = orc.search_batch(oix, qs, qe, mode=0)
+    off, ks, ke = ix.search_keys(qs, qe)
+    assert np.array_equal(off, ooff.astype(np.uint64))
+    assert np.array_equal(ks.astype(np.int64), oix["starts"][oidx].astype(np.int64))
+    assert np.array_equal(ke.astype(np.int64), oix["ends"][oidx].astype(np.int64))
+    off2, ks2, ke2, kv2 = ix.search_keys(qs, qe, items=True)
+    assert np.array_equal(off2, off) and np.array_equal(ks2, ks) and np.array_equal(ke2, ke)
+    assert np.array_equal(kv2, oix["data"][oidx])
+    off0, ks0, ke0 = ix.search_keys(qs[:0], qe[:0])
+    assert off0.tolist() == [0] and ks0.size == 0
+    ix.close()
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
 def test_planned_batch_reuse(sia, orc, dtype):
     """One plan serves count, offsets and every fill mode, repeatedly, on one stream."""
     import torch
