@@ -1,11 +1,14 @@
 """Autotuner for the walk kernels' launch choices.
 
 The kernels' results never depend on these choices (tile size, resident workgroups per
-CU for the count and the fill pass); only their speed does.  tune() times every
-candidate on the caller's own index and planned batch with HIP events and keeps the
-fastest count + fill combination.  Results are cached per workload signature (coordinate
-width, log2 of the index size, log2 of the batch size, log2 of the mean hits per query)
-in a JSON file, so later runs of the same shape reuse them.
+CU for the count and the fill pass, the per-lane step budget before a long walk is
+parked for the one-wave-per-query kernel, walk or rank form of the count); only their
+speed does.  tune() times every candidate on the caller's own index and planned batch
+with HIP events: first tile x workgroups for count + fill, then, with those fixed, the
+lane step budget by the fill's time and the count form by the count's.  Results are
+cached per workload signature (coordinate width, log2 of the index size, log2 of the
+batch size, log2 of the mean hits per query) in a JSON file, so later runs of the same
+shape reuse them.
 """
 from __future__ import annotations
 
@@ -20,6 +23,7 @@ DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "superintervals_
 KNOBS = ("tile", "count_wgs", "fill_wgs", "count_rank", "offsets_rank", "lane_steps", "plan_sync", "ub_tree")
 CANDIDATE_TILES = (512, 1024, 2048, 4096)
 CANDIDATE_WGS = (1, 0)     # 0 = the occupancy limit
+CANDIDATE_LANE_STEPS = (64, 256, 1024)
 
 
 def current() -> dict:
@@ -62,8 +66,21 @@ def _save(path, table):
     os.replace(tmp, path)
 
 
+def _time_ms(fn, reps, stream):
+    """Mean ms of fn() over reps calls on stream (HIP events), after one warm call."""
+    import torch
+    fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
 def tune(ix, plan_t, nq, offsets_t, idx_t, stream=None, reps=3, tiles=CANDIDATE_TILES,
-         wgs=CANDIDATE_WGS, verbose=False):
+         wgs=CANDIDATE_WGS, lane_steps=CANDIDATE_LANE_STEPS, verbose=False):
     """Time count + fill for every candidate on a planned batch; apply and return the best.
 
     ix is a DeviceIndex, plan_t a planned batch (plan_device), offsets_t the batch's
@@ -73,42 +90,53 @@ def tune(ix, plan_t, nq, offsets_t, idx_t, stream=None, reps=3, tiles=CANDIDATE_
     stream = stream or torch.cuda.current_stream()
     counts = torch.empty(nq, dtype=torch.int32, device=offsets_t.device)
     saved = current()
+    count = lambda: ix.count_planned(plan_t, nq, counts, stream=stream)
+    fill = lambda: ix.fill_planned(plan_t, nq, offsets_t, idx_t, stream=stream)
     results = []
-    for tile in tiles:
-        for cw in wgs:
-            for fw in wgs:
-                apply({"tile": tile, "count_wgs": cw, "fill_wgs": fw})
-                ix.count_planned(plan_t, nq, counts, stream=stream)       # warm
-                ix.fill_planned(plan_t, nq, offsets_t, idx_t, stream=stream)
-                e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
-                e0.record(stream)
-                for _ in range(reps):
-                    ix.count_planned(plan_t, nq, counts, stream=stream)
-                e1.record(stream)
-                for _ in range(reps):
-                    ix.fill_planned(plan_t, nq, offsets_t, idx_t, stream=stream)
-                e2.record(stream)
-                e2.synchronize()
-                tc, tf = e0.elapsed_time(e1) / reps, e1.elapsed_time(e2) / reps
-                results.append({"tile": tile, "count_wgs": cw, "fill_wgs": fw, "count_ms": tc, "fill_ms": tf})
-                if verbose:
-                    print("tune tile=%d count_wgs=%d fill_wgs=%d: count %.3f ms fill %.3f ms" % (tile, cw, fw, tc, tf),
-                          flush=True)
-    apply(saved)
-    best_count = min(results, key=lambda r: r["count_ms"])
-    # count and fill share the tile; pick the tile by their sum, the wgs of each by its own time
-    by_tile = {}
-    for r in results:
-        c = by_tile.setdefault(r["tile"], {"count": r, "fill": r})
-        if r["count_ms"] < c["count"]["count_ms"]:
-            c["count"] = r
-        if r["fill_ms"] < c["fill"]["fill_ms"]:
-            c["fill"] = r
-    tile = min(by_tile, key=lambda t: by_tile[t]["count"]["count_ms"] + by_tile[t]["fill"]["fill_ms"])
-    best = {"tile": tile, "count_wgs": by_tile[tile]["count"]["count_wgs"],
-            "fill_wgs": by_tile[tile]["fill"]["fill_wgs"],
-            "count_ms": by_tile[tile]["count"]["count_ms"], "fill_ms": by_tile[tile]["fill"]["fill_ms"]}
-    del best_count
+    try:
+        for tile in tiles:
+            for cw in wgs:
+                for fw in wgs:
+                    apply({"tile": tile, "count_wgs": cw, "fill_wgs": fw})
+                    tc, tf = _time_ms(count, reps, stream), _time_ms(fill, reps, stream)
+                    results.append({"tile": tile, "count_wgs": cw, "fill_wgs": fw, "count_ms": tc, "fill_ms": tf})
+                    if verbose:
+                        print("tune tile=%d count_wgs=%d fill_wgs=%d: count %.3f ms fill %.3f ms"
+                              % (tile, cw, fw, tc, tf), flush=True)
+        # count and fill share the tile; pick the tile by their sum, the wgs of each by its own time
+        by_tile = {}
+        for r in results:
+            c = by_tile.setdefault(r["tile"], {"count": r, "fill": r})
+            if r["count_ms"] < c["count"]["count_ms"]:
+                c["count"] = r
+            if r["fill_ms"] < c["fill"]["fill_ms"]:
+                c["fill"] = r
+        tile = min(by_tile, key=lambda t: by_tile[t]["count"]["count_ms"] + by_tile[t]["fill"]["fill_ms"])
+        best = {"tile": tile, "count_wgs": by_tile[tile]["count"]["count_wgs"],
+                "fill_wgs": by_tile[tile]["fill"]["fill_wgs"],
+                "count_ms": by_tile[tile]["count"]["count_ms"], "fill_ms": by_tile[tile]["fill"]["fill_ms"]}
+        apply(knobs_of(best))
+        # the lane step budget (shared by the count walk and the fills), by the fill's time
+        for ls in lane_steps:
+            apply({"lane_steps": ls})
+            tf = _time_ms(fill, reps, stream)
+            results.append(dict(knobs_of(best), lane_steps=ls, fill_ms=tf))
+            if verbose:
+                print("tune lane_steps=%d: fill %.3f ms" % (ls, tf), flush=True)
+            if "lane_steps" not in best or tf < best["fill_ms"]:
+                best["lane_steps"], best["fill_ms"] = ls, tf
+        apply({"lane_steps": best["lane_steps"]})
+        # the count's form: the walk (best for short lists) or the rank form (O(log n))
+        for cr in (0, 1):
+            apply({"count_rank": cr})
+            tc = _time_ms(count, reps, stream)
+            results.append(dict(knobs_of(best), count_rank=cr, count_ms=tc))
+            if verbose:
+                print("tune count_rank=%d: count %.3f ms" % (cr, tc), flush=True)
+            if "count_rank" not in best or tc < best["count_ms"]:
+                best["count_rank"], best["count_ms"] = cr, tc
+    finally:
+        apply(saved)
     apply(knobs_of(best))
     return best, results
 

@@ -542,9 +542,18 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
 }
 
 // Wave-cooperative continuation of parked walks: one wave per query, one aligned line of
-// 64 ends per step (a coalesced 256-byte load for int32), hits by ballot.  The line exit
-// is the same rule as the lane walk's, with the leftmost maximum found by a wave max
-// reduction and a ballot.  Output is written at each hit's rank among the line's hits.
+// 64 ends per step (a coalesced 256-byte load for int32), hits by ballot.  Output is
+// written at each hit's rank among the line's hits.  One round trip fetches the line
+// holding the walk position and the HEAVY_LINES - 1 lines below it, ends and exitp
+// together, into registers; the walk then steps through them (parked walks are long and
+// visit most lines of their span) and fetches again only when it leaves them.
+//
+// Line exit.  With the trailing misses [b0, b0 + x] of the line (b0 not a hit), the walk
+// leaves through exitp[b0 + x], which is the exit of the 64-byte sub-line of exitp's own
+// width holding b0 + x.  A target still at or above b0 is a miss of a lower sub-line whose
+// entries below it are misses too, so its own exitp entry applies again: at most 64 / W
+// register lookups (readlane) give the first position below b0 that the reference walk
+// reaches, with no dependent memory access.
 //
 // All 64 lanes must reach every ballot and shuffle together.  The values that steer the
 // loops (entry number, walk position) are therefore made wave-uniform with readfirstlane,
@@ -552,6 +561,8 @@ __global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S>
 // lane-0-only block (a lane-0 store at the end of the body) the compiler merged the two
 // lane == 0 tests and sent lanes 1..63 back to the shuffle without lane 0; they read the
 // disabled lane, re-walked entry 0 and took an empty ballot's -1 as a line offset.
+constexpr uint32_t HEAVY_LINES = 4;   // 64-position lines a heavy walk fetches per round trip
+
 template <typename S, int MODE>
 __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanView<S> pv,
                                                           uint32_t* __restrict__ counts,
@@ -578,10 +589,34 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanV
         bool lead = __builtin_amdgcn_readfirstlane(he.lead) != 0;
         uint64_t acc = he.acc;
         const uint64_t oend = (MODE != W_COUNT) ? min(offsets[q + 1], cap) : 0;
+        uint32_t tb = NONE;   // base of the top fetched line (NONE: nothing fetched yet)
+        S cv[HEAVY_LINES];
+        uint32_t cx[HEAVY_LINES];
         while (cur != NONE) {
             const uint32_t b0 = cur & ~63u;
             const uint32_t kc = cur - b0;
-            const S v = ix.ends[b0 + lane];
+            // the walk only moves down, so tb >= b0 once fetched
+            uint32_t j = (tb == NONE) ? HEAVY_LINES : (tb - b0) >> 6;
+            if (j >= HEAVY_LINES) {
+                tb = b0;
+                j = 0;
+#pragma unroll
+                for (uint32_t l = 0; l < HEAVY_LINES; ++l) {
+                    // lines below position 0 are never stepped into; line 0 stands in for them
+                    const uint32_t lb = (b0 >= 64u * l) ? b0 - 64u * l : 0u;
+                    cv[l] = ix.ends[lb + lane];
+                    cx[l] = ix.exitp[lb + lane];
+                }
+            }
+            S v = cv[0];
+            uint32_t ex = cx[0];
+#pragma unroll
+            for (uint32_t l = 1; l < HEAVY_LINES; ++l) {
+                if (j == l) {
+                    v = cv[l];
+                    ex = cx[l];
+                }
+            }
             const bool hit = ((uint32_t)lane <= kc) && (v >= s);
             const uint64_t m = __ballot(hit);
             if (MODE == W_VECTOR && lead) {
@@ -621,14 +656,10 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanV
                 cur = b0 - 1u;
             } else {
                 const uint32_t x = m ? (uint32_t)(__builtin_ctzll(m) - 1) : kc;
-                const bool cand = (uint32_t)lane <= x;
-                S best = cand ? v : ((sizeof(S) == 4) ? (S)INT32_MIN : (S)INT64_MIN);
-                best = wave_max(best);
-                const uint64_t am = __ballot(cand && v == best);
-                // lane 0 is a candidate whenever b0 is a miss, so am != 0 with all lanes
-                // active; the guard keeps the offset inside the line regardless
-                const uint32_t a = am ? (uint32_t)__builtin_ctzll(am) : 0u;
-                cur = __builtin_amdgcn_readfirstlane(ix.branch[b0 + a]);
+                uint32_t p = (uint32_t)__builtin_amdgcn_readlane((int)ex, (int)x);
+                // each lookup moves to a lower sub-line of this line, or leaves it
+                while (p != NONE && p >= b0) p = (uint32_t)__builtin_amdgcn_readlane((int)ex, (int)(p - b0));
+                cur = p;
             }
         }
         if (MODE == W_COUNT) {

@@ -280,6 +280,80 @@ def test_heavy_path_parity(sia, orc, lane_steps):
     ix.close()
 
 
+@pytest.mark.parametrize("lane_steps", [1, 40])
+def test_heavy_path_int64_long_walks(sia, orc, lane_steps):
+    """int64 hotspot mixture (workload 3 distributions): every walk parked after lane_steps
+    lines and finished by the wave-cooperative kernel, which fetches four lines at a time
+    and leaves a line through exitp lookups.  Counts, both orders and values equal the
+    oracle's."""
+    from superintervals_amd import autotune
+    print(f"[gpu] heavy path int64 lane_steps={lane_steps}", flush=True)
+    s, e = sia.workload_intervals(3, 1_000_000, seed=33, sorted=False)
+    qs, qe = sia.workload_queries(3, 6_000, seed=34)
+    # ragged edges: queries before, after and around the whole index, and a point query
+    qs = np.concatenate([qs, np.array([-5, 0, int(e.max()), int(s.min()), 10**9], np.int64)])
+    qe = np.concatenate([qe, np.array([-1, int(e.max()) + 10, int(e.max()) + 10, int(s.min()), 10**9], np.int64)])
+    vals = (np.arange(s.size, dtype=np.int64) * 7 % 1_000_003).astype(np.int32)
+    ix = _dev_index(sia, s, e, vals)
+    oix = orc.build_index(s, e, vals)
+    saved = autotune.current()
+    try:
+        autotune.apply({"lane_steps": lane_steps, "offsets_rank": 0, "count_rank": 0})
+        ocounts = orc.count_batch(oix, qs, qe)
+        assert np.array_equal(ix.count(qs, qe), ocounts)
+        for order, mode in (("desc", 0), ("vector", 1)):
+            off, idx = ix.search(qs, qe, order=order)
+            ooff, oidx = orc.search_batch(oix, qs, qe, mode=mode)
+            assert np.array_equal(off, ooff), order
+            assert np.array_equal(idx.astype(np.int64), oidx), order
+        off, v = ix.search(qs, qe, values=True)
+        ooff, oidx = orc.search_batch(oix, qs, qe, mode=0)
+        assert np.array_equal(v, oix["data"][oidx])
+        # a walk with more than 4 x 64 hits spans more lines than one fetch holds
+        assert (ocounts > 4 * 64).sum() > 100
+    finally:
+        autotune.apply(saved)
+    ix.close()
+
+
+def test_autotune_picks_and_keeps_results(sia, orc):
+    """tune() on a planned batch picks tile, residency, lane step budget and count form,
+    applies them, and the library's output with them equals the oracle's."""
+    import torch
+    from superintervals_amd import autotune
+    print("[gpu] autotune on workload 3 (300K int64 intervals)", flush=True)
+    s, e = sia.workload_intervals(3, 300_000, seed=35, sorted=False)
+    qs, qe = sia.workload_queries(3, 4_000, seed=36)
+    ix = _dev_index(sia, s, e)
+    oix = orc.build_index(s, e)
+    dev = torch.device("cuda:0")
+    tqs, tqe = torch.from_numpy(qs).to(dev), torch.from_numpy(qe).to(dev)
+    nq = qs.size
+    plan = torch.empty(ix.plan_bytes(nq), dtype=torch.uint8, device=dev)
+    off = torch.empty(nq + 1, dtype=torch.int64, device=dev)
+    saved = autotune.current()
+    try:
+        ix.plan_device(tqs, tqe, plan)
+        ix.offsets_planned(plan, nq, off)
+        total = int(off[-1].item())
+        idx = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+        best, table = autotune.tune(ix, plan, nq, off, idx, reps=1, tiles=(512, 2048), wgs=(0,),
+                                    lane_steps=(16, 256))
+        assert best["lane_steps"] in (16, 256) and best["count_rank"] in (0, 1)
+        now = autotune.current()
+        for k, v in autotune.knobs_of(best).items():
+            assert now[k] == v, k
+        assert len(table) == 2 + 2 + 2
+        counts = ix.count(qs, qe)
+        assert np.array_equal(counts, orc.count_batch(oix, qs, qe))
+        o, i = ix.search(qs, qe, order="desc")
+        ooff, oidx = orc.search_batch(oix, qs, qe, mode=0)
+        assert np.array_equal(o, ooff) and np.array_equal(i.astype(np.int64), oidx)
+    finally:
+        autotune.apply(saved)
+    ix.close()
+
+
 def test_workload3_wide_queries(sia, orc):
     """int64 hotspot mixture with 1 % wide queries (workload 3 distributions, 2M intervals),
     unsorted input: full CSR against the oracle in both orders."""
