@@ -553,7 +553,11 @@ def main():
     t_fill = sum(ev[2].elapsed_time(ev[3]) for ev in evs) / a.steps
     ms_step = wall * 1e3 / a.steps
 
-    # count kernel alone on the planned batch (one launch per pass)
+    # count walk alone on the planned batch (one launch per pass); the walk form is forced
+    # here whatever the tuner chose, since the byte model below is the walk's
+    from superintervals_amd import autotune
+    tuned_count_rank = autotune.current()["count_rank"]
+    autotune.apply({"count_rank": 0})
     c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     for _ in range(2):
         ix.count_planned(plan, nq, counts, stream=stream)
@@ -564,7 +568,6 @@ def main():
     torch.cuda.synchronize()
     t_count = c0.elapsed_time(c1) / a.steps
     # rank form of count (upper_bound from the plan + one search of the sorted ends)
-    from superintervals_amd import autotune
     autotune.apply({"count_rank": 1})
     ix.count_planned(plan, nq, counts, stream=stream)
     c0.record(stream)
@@ -572,7 +575,7 @@ def main():
         ix.count_planned(plan, nq, counts, stream=stream)
     c1.record(stream)
     torch.cuda.synchronize()
-    autotune.apply({"count_rank": 0})
+    autotune.apply({"count_rank": tuned_count_rank})
     t_count_rank = c0.elapsed_time(c1) / a.steps
 
     # Algorithmic bytes (per query: c = coordinate bytes, V/J = the reference walk's ends
@@ -619,7 +622,9 @@ def main():
                    "launch": tuning if tuning else "library defaults"},
         "roofline": roof,
         "kernels": kernels,
-        "count_queries_per_s": nq / (t_count / 1e3),
+        # count() as tuned (walk or rank form), and the walk alone
+        "count_queries_per_s": nq / ((t_count_rank if tuned_count_rank else t_count) / 1e3),
+        "count_walk_queries_per_s": nq / (t_count / 1e3),
         "walk": {"hits": hits, "ends_reads": V, "branch_reads": J, "found": found},
     }
     if not a.no_cpu:

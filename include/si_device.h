@@ -112,7 +112,7 @@ double siIndexLastBuildMs(const siIndex* ix);
 
 /* ---- Launch tuning ----------------------------------------------------------
  * Process-wide launch choices of the walk kernels (results never depend on them):
- *   "tile"       sorted queries per workgroup tile (64 .. 2^20, default 2048)
+ *   "tile"       sorted queries per workgroup tile (64 .. 2^20, default 1024)
  *   "count_wgs"  resident workgroups per CU in the count pass (0 = occupancy limit)
  *   "fill_wgs"   resident workgroups per CU in the fill passes (0 = occupancy limit)
  *   "count_rank"   1: siCount* use the rank form (upper_bound + 1 - #{ends < s}, exact

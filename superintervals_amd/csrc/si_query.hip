@@ -208,7 +208,7 @@ __global__ __launch_bounds__(Q_THREADS) void k_count_rank(IndexView<S> ix, const
 //                digits (one stream synchronisation); 0: it sorts every key digit without
 //                a host read-back, so plan -> offsets -> fill can be captured in a graph
 struct Tuning {
-    uint32_t tile = 2048;
+    uint32_t tile = 1024;
     int count_wgs = 0;
     int fill_wgs = 0;
     int count_rank = 0;
@@ -362,15 +362,32 @@ __device__ __forceinline__ void emit_segments(uint32_t c, uint32_t m, uint32_t b
 }
 
 // LDS window of a tile: ends and exitp of positions [wlo, wlo + WIN), which always
-// ends at or above the tile's largest upper_bound.  64 KB per workgroup.
+// ends at or above the tile's largest upper_bound.  26 KB (int32) / 28 KB (int64) per
+// workgroup, so that with the 12 KB of emission slots four 512-thread workgroups fit in
+// a CU's 160 KB of LDS and the walk runs 8 waves per SIMD (its registers are held under
+// 64 for that).  On 10M int32 intervals x 10M queries this beats an 8128-entry window at
+// 4 waves per SIMD by 20 % (fill) with tiles of 1024 queries.  The sizes (multiples of
+// 16) and the waves per SIMD can be set at compile time (SI_WALK_WIN32, SI_WALK_WIN64,
+// SI_WALK_WAVES_PER_EU).
+#ifndef SI_WALK_WIN32
+#define SI_WALK_WIN32 3328
+#endif
+#ifndef SI_WALK_WIN64
+#define SI_WALK_WIN64 2368
+#endif
+#ifndef SI_WALK_WAVES_PER_EU
+#define SI_WALK_WAVES_PER_EU 8
+#endif
+#define SI_WALK_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SI_WALK_WAVES_PER_EU, SI_WALK_WAVES_PER_EU)))
 template <typename S> struct TileCfg;
-template <> struct TileCfg<int32_t> { static constexpr uint32_t WIN = 8128; };   // 8128 x 8 B
-template <> struct TileCfg<int64_t> { static constexpr uint32_t WIN = 5376; };   // 5376 x 12 B
+template <> struct TileCfg<int32_t> { static constexpr uint32_t WIN = SI_WALK_WIN32; };   // x 8 B (ends, exitp)
+template <> struct TileCfg<int64_t> { static constexpr uint32_t WIN = SI_WALK_WIN64; };   // x 12 B
+static_assert(SI_WALK_WIN32 % 16 == 0 && SI_WALK_WIN64 % 16 == 0, "window sizes are multiples of 16");
 
 constexpr int T_THREADS = 512;   // 8 waves per workgroup, two workgroups per CU
 
 template <typename S, int MODE>
-__global__ __launch_bounds__(T_THREADS) void k_walk(IndexView<S> ix, PlanView<S> pv, uint32_t nq,
+__global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<S> ix, PlanView<S> pv, uint32_t nq,
                                                     uint32_t tile, uint32_t ntiles, uint32_t lane_steps,
                                                     uint32_t* __restrict__ tile_counter,
                                                     uint32_t* __restrict__ counts,
