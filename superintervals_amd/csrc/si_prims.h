@@ -42,6 +42,30 @@ __device__ __forceinline__ T wave_incl_scan(T v) {
     return v;
 }
 
+// Inclusive scan of a 32-bit value across the wave in six DPP adds: shifts by 1, 2, 4, 8
+// within each row of 16 lanes (a lane with no source in its row adds 0, bound_ctrl), then
+// row 0's total broadcast into rows 1 and 3 (row_bcast:15, rows 0 and 2 not written) and
+// lane 31's into rows 2 and 3 (row_bcast:31).  A DPP source VGPR written by the previous
+// VALU instruction needs two wait states, hence the s_nop 1 before each add.
+__device__ __forceinline__ uint32_t wave_incl_scan_dpp(uint32_t v) {
+    asm volatile(
+        "s_nop 1\n\t"
+        "v_add_u32_dpp %0, %0, %0 row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:0\n\t"
+        "s_nop 1\n\t"
+        "v_add_u32_dpp %0, %0, %0 row_shr:2 row_mask:0xf bank_mask:0xf bound_ctrl:0\n\t"
+        "s_nop 1\n\t"
+        "v_add_u32_dpp %0, %0, %0 row_shr:4 row_mask:0xf bank_mask:0xf bound_ctrl:0\n\t"
+        "s_nop 1\n\t"
+        "v_add_u32_dpp %0, %0, %0 row_shr:8 row_mask:0xf bank_mask:0xf bound_ctrl:0\n\t"
+        "s_nop 1\n\t"
+        "v_add_u32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+        "s_nop 1\n\t"
+        "v_add_u32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+        "s_nop 1"
+        : "+v"(v));
+    return v;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_max(T v) {
 #pragma unroll

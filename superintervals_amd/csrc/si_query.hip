@@ -310,7 +310,7 @@ __device__ __forceinline__ void emit_segments(uint32_t c, uint32_t m, uint32_t b
                                               int32_t* __restrict__ out_val) {
     constexpr int W = Line<S>::W;
     if (__ballot(c != 0) == 0) return;
-    const uint32_t inc = wave_incl_scan(c);
+    const uint32_t inc = wave_incl_scan_dpp(c);
     const uint32_t total = __builtin_amdgcn_readlane(inc, WAVE - 1);
     const uint32_t pre = inc - c;
     const uint64_t le = (lane == WAVE - 1) ? ~0ull : ((2ull << lane) - 1ull);   // lanes <= this
@@ -578,7 +578,10 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
 // lane-0-only block (a lane-0 store at the end of the body) the compiler merged the two
 // lane == 0 tests and sent lanes 1..63 back to the shuffle without lane 0; they read the
 // disabled lane, re-walked entry 0 and took an empty ballot's -1 as a line offset.
-constexpr uint32_t HEAVY_LINES = 4;   // 64-position lines a heavy walk fetches per round trip
+#ifndef SI_HEAVY_LINES
+#define SI_HEAVY_LINES 4
+#endif
+constexpr uint32_t HEAVY_LINES = SI_HEAVY_LINES;   // 64-position lines a heavy walk fetches per round trip
 
 template <typename S, int MODE>
 __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanView<S> pv,
@@ -606,77 +609,73 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanV
         bool lead = __builtin_amdgcn_readfirstlane(he.lead) != 0;
         uint64_t acc = he.acc;
         const uint64_t oend = (MODE != W_COUNT) ? min(offsets[q + 1], cap) : 0;
-        uint32_t tb = NONE;   // base of the top fetched line (NONE: nothing fetched yet)
-        S cv[HEAVY_LINES];
-        uint32_t cx[HEAVY_LINES];
         while (cur != NONE) {
-            const uint32_t b0 = cur & ~63u;
-            const uint32_t kc = cur - b0;
-            // the walk only moves down, so tb >= b0 once fetched
-            uint32_t j = (tb == NONE) ? HEAVY_LINES : (tb - b0) >> 6;
-            if (j >= HEAVY_LINES) {
-                tb = b0;
-                j = 0;
+            // fetch the line holding cur and the lines below it
+            const uint32_t tb = cur & ~63u;
+            S cv[HEAVY_LINES];
+            uint32_t cx[HEAVY_LINES];
 #pragma unroll
-                for (uint32_t l = 0; l < HEAVY_LINES; ++l) {
-                    // lines below position 0 are never stepped into; line 0 stands in for them
-                    const uint32_t lb = (b0 >= 64u * l) ? b0 - 64u * l : 0u;
-                    cv[l] = ix.ends[lb + lane];
-                    cx[l] = ix.exitp[lb + lane];
-                }
+            for (uint32_t l = 0; l < HEAVY_LINES; ++l) {
+                // lines below position 0 are never stepped into; line 0 stands in for them
+                const uint32_t lb = (tb >= 64u * l) ? tb - 64u * l : 0u;
+                cv[l] = ix.ends[lb + lane];
+                cx[l] = ix.exitp[lb + lane];
             }
-            S v = cv[0];
-            uint32_t ex = cx[0];
+            // the walk leaves every line it steps into and only moves down, so each fetched
+            // line is stepped into at most once, in this order (a line below position 0
+            // has a wrapped base that no position matches)
 #pragma unroll
-            for (uint32_t l = 1; l < HEAVY_LINES; ++l) {
-                if (j == l) {
-                    v = cv[l];
-                    ex = cx[l];
-                }
-            }
-            const bool hit = ((uint32_t)lane <= kc) && (v >= s);
-            const uint64_t m = __ballot(hit);
-            if (MODE == W_VECTOR && lead) {
-                const uint64_t lim = (kc == 63) ? ~0ull : ((2ull << kc) - 1ull);
-                const uint64_t inv = ~m & lim;
-                uint32_t lo;
-                if (inv == 0) {
-                    cur = b0 - 1u;
-                    lo = 0;
-                } else {
-                    const uint32_t hm = 63u - (uint32_t)__builtin_clzll(inv);
-                    cur = b0 + hm;
-                    lo = cur + 1u;
-                }
-                if (inv != 0 || cur == NONE) {
-                    const uint64_t c = min((uint64_t)(top - lo + 1u), oend - acc);
-                    for (uint64_t x = lane; x < c; x += WAVE) out_idx[acc + x] = lo + (uint32_t)x;
-                    acc += c;
-                    lead = false;
-                }
-                continue;
-            }
-            const uint32_t nh = (uint32_t)__popcll(m);
-            if (MODE == W_COUNT) {
-                acc += nh;
-            } else {
-                if (hit) {
-                    const uint64_t r = (uint64_t)__popcll(m & ~le_mask);   // hits above this lane
-                    if (acc + r < oend) {
-                        if (MODE == W_VALUES) out_val[acc + r] = vals[b0 + lane];
-                        else out_idx[acc + r] = b0 + (uint32_t)lane;
+            for (uint32_t l = 0; l < HEAVY_LINES; ++l) {
+                const uint32_t b0 = tb - 64u * l;
+                if (cur == NONE || (cur & ~63u) != b0) continue;
+                const uint32_t kc = cur - b0;
+                const S v = cv[l];
+                const uint32_t ex = cx[l];
+                const bool hit = ((uint32_t)lane <= kc) && (v >= s);
+                const uint64_t m = __ballot(hit);
+                if (MODE == W_VECTOR && lead) {
+                    const uint64_t lim = (kc == 63) ? ~0ull : ((2ull << kc) - 1ull);
+                    const uint64_t inv = ~m & lim;
+                    uint32_t lo;
+                    if (inv == 0) {
+                        cur = b0 - 1u;
+                        lo = 0;
+                    } else {
+                        const uint32_t hm = 63u - (uint32_t)__builtin_clzll(inv);
+                        cur = b0 + hm;   // the first miss below the run: this line again
+                        lo = cur + 1u;
                     }
+                    if (inv != 0 || cur == NONE) {
+                        const uint64_t c = min((uint64_t)(top - lo + 1u), oend - acc);
+                        for (uint64_t x = lane; x < c; x += WAVE) out_idx[acc + x] = lo + (uint32_t)x;
+                        acc += c;
+                        lead = false;
+                    }
+                    if (inv != 0) break;   // the walk resumes inside this line: fetch again
+                    continue;
                 }
-                acc = min(acc + nh, oend);
-            }
-            if (m & 1ull) {
-                cur = b0 - 1u;
-            } else {
-                const uint32_t x = m ? (uint32_t)(__builtin_ctzll(m) - 1) : kc;
-                uint32_t p = (uint32_t)__builtin_amdgcn_readlane((int)ex, (int)x);
-                // each lookup moves to a lower sub-line of this line, or leaves it
-                while (p != NONE && p >= b0) p = (uint32_t)__builtin_amdgcn_readlane((int)ex, (int)(p - b0));
-                cur = p;
+                const uint32_t nh = (uint32_t)__popcll(m);
+                if (MODE == W_COUNT) {
+                    acc += nh;
+                } else {
+                    if (hit) {
+                        const uint64_t r = (uint64_t)__popcll(m & ~le_mask);   // hits above this lane
+                        if (acc + r < oend) {
+                            if (MODE == W_VALUES) out_val[acc + r] = vals[b0 + lane];
+                            else out_idx[acc + r] = b0 + (uint32_t)lane;
+                        }
+                    }
+                    acc = min(acc + nh, oend);
+                }
+                if (m & 1ull) {
+                    cur = b0 - 1u;
+                } else {
+                    const uint32_t x = m ? (uint32_t)(__builtin_ctzll(m) - 1) : kc;
+                    uint32_t p = (uint32_t)__builtin_amdgcn_readlane((int)ex, (int)x);
+                    // each lookup moves to a lower sub-line of this line, or leaves it
+                    while (p != NONE && p >= b0) p = (uint32_t)__builtin_amdgcn_readlane((int)ex, (int)(p - b0));
+                    cur = p;
+                }
             }
         }
         if (MODE == W_COUNT) {
