@@ -302,9 +302,27 @@ __device__ __forceinline__ void wave_lds_sync() { __builtin_amdgcn_fence(__ATOMI
 // and slot x takes the last start at or before x (or the segment running in from the
 // previous chunk).  A store happens only at r < c of its segment, so no slot is written
 // outside the segment its owner was given.  All lanes execute the ballots.
+#ifndef SI_EMIT_LUT
+#define SI_EMIT_LUT 1
+#endif
+
+// Table of the set bits of a byte, highest first: nibble k of kbit_entry(b) is the bit
+// position of the (k + 1)-th highest set bit of b (unused nibbles are 0).
+__device__ __forceinline__ uint32_t kbit_entry(uint32_t b) {
+    uint32_t e = 0, k = 0;
+    for (int p = 7; p >= 0; --p) {
+        if ((b >> p) & 1u) {
+            e |= (uint32_t)p << (4u * k);
+            ++k;
+        }
+    }
+    return e;
+}
+
 template <typename S, int MODE>
 __device__ __forceinline__ void emit_segments(uint32_t c, uint32_t m, uint32_t base, bool asc,
                                               uint64_t o, int lane, EmitSlot* __restrict__ sd,
+                                              const uint32_t* __restrict__ kbit,
                                               uint32_t* __restrict__ out_idx,
                                               const int32_t* __restrict__ vals,
                                               int32_t* __restrict__ out_val) {
@@ -345,6 +363,19 @@ __device__ __forceinline__ void emit_segments(uint32_t c, uint32_t m, uint32_t b
         uint32_t pos;
         if ((d.kv >> 16) & 1u) {
             pos = d.base + r;
+        } else if (SI_EMIT_LUT) {
+            // the (r+1)-th highest set bit of om, from the byte table (r >= c only happens
+            // on slots that store nothing)
+            uint32_t byte = om & 0xFFu, k = r, add = 0;
+            if (W == 16) {
+                const uint32_t hi = om >> 8;
+                const uint32_t ph = (uint32_t)__popc(hi);
+                const bool up = r < ph;
+                byte = up ? hi : byte;
+                k = up ? r : r - ph;
+                add = up ? 8u : 0u;
+            }
+            pos = d.base + add + ((kbit[byte] >> ((4u * k) & 31u)) & 7u);
         } else {
             // the (r+1)-th highest set bit of om: largest p with popc(om >> p) >= r + 1
             uint32_t p = 0;
@@ -362,10 +393,10 @@ __device__ __forceinline__ void emit_segments(uint32_t c, uint32_t m, uint32_t b
 }
 
 // LDS window of a tile: ends and exitp of positions [wlo, wlo + WIN), which always
-// ends at or above the tile's largest upper_bound.  26 KB (int32) / 28 KB (int64) per
-// workgroup, so that with the 12 KB of emission slots four 512-thread workgroups fit in
-// a CU's 160 KB of LDS and the walk runs 8 waves per SIMD (its registers are held under
-// 64 for that).  On 10M int32 intervals x 10M queries this beats an 8128-entry window at
+// ends at or above the tile's largest upper_bound.  26 KB (int32) / 27 KB (int64) per
+// workgroup, so that with the 12 KB of emission slots and the 1 KB bit table four
+// 512-thread workgroups fit in a CU's 160 KB of LDS and the walk runs 8 waves per SIMD
+// (its registers are held under 64 for that).  On 10M int32 intervals x 10M queries this beats an 8128-entry window at
 // 4 waves per SIMD by 20 % (fill) with tiles of 1024 queries.  The sizes (multiples of
 // 16) and the waves per SIMD can be set at compile time (SI_WALK_WIN32, SI_WALK_WIN64,
 // SI_WALK_WAVES_PER_EU).
@@ -373,7 +404,7 @@ __device__ __forceinline__ void emit_segments(uint32_t c, uint32_t m, uint32_t b
 #define SI_WALK_WIN32 3328
 #endif
 #ifndef SI_WALK_WIN64
-#define SI_WALK_WIN64 2368
+#define SI_WALK_WIN64 2288
 #endif
 #ifndef SI_WALK_WAVES_PER_EU
 #define SI_WALK_WAVES_PER_EU 8
@@ -402,9 +433,13 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
     __shared__ __align__(16) uint32_t s_ex[WIN];   // exit targets (IndexView::exitp)
     __shared__ uint32_t s_tile, s_next;
     __shared__ EmitSlot s_emit[(MODE == W_COUNT) ? 1 : T_THREADS];   // per-wave emission slots
+    __shared__ uint32_t s_kbit[(MODE == W_COUNT || !SI_EMIT_LUT) ? 1 : 256];   // byte set-bit table
     const int lane = lane_id();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     if (MODE != W_COUNT) s_emit[threadIdx.x].kv = 0u;   // emission slots start free (own wave's)
+    if (MODE != W_COUNT && SI_EMIT_LUT) {
+        for (uint32_t b = threadIdx.x; b < 256u; b += T_THREADS) s_kbit[b] = kbit_entry(b);
+    }   // read only after the first tile's __syncthreads
 
     while (true) {
         // ---- take the next tile (workgroup-level work stealing)
@@ -530,7 +565,7 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
                 }
             }
             if (MODE != W_COUNT) {
-                emit_segments<S, MODE>(c_emit, e_m, e_base, e_asc, e_o, lane, s_emit + (threadIdx.x & ~63u),
+                emit_segments<S, MODE>(c_emit, e_m, e_base, e_asc, e_o, lane, s_emit + (threadIdx.x & ~63u), s_kbit,
                                        out_idx, vals, out_val);
             }
             if (has && cur == NONE) {
