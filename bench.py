@@ -362,7 +362,7 @@ def bench_skewed(a):
     out = {"metric": "skewed int64 batch: count queries/s (one launch) and streamed search_idxs positions/s",
            "value": nq / (t_walk / 1e3), "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
            "dtype": "int64", "data": "synthetic (seeded hotspot-mixture generator, inputs resident in HBM)",
-           "config": {"workload": "50M int64 intervals (hotspot mixture, 20 %% long) x 20M queries (1 %% wide)",
+           "config": {"workload": "%dM int64 intervals (hotspot mixture, 20 %% long) x %dM queries (1 %% wide)" % (n // 10**6, nq // 10**6),
                       "intervals": n, "queries": nq, "search_batch": bq, "host_generation_s": round(gen_s, 2),
                       "build_ms": round(ix.last_build_ms, 2)},
            "plan_ms": t_plan, "count_walk_ms": t_walk, "count_rank_ms": t_rank, "count_identical": same,
