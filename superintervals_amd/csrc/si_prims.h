@@ -208,8 +208,8 @@ constexpr int RS_ROUNDS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;   // 4096 items per tile
 constexpr int RS_BINS = 256;
 
-// Histograms of all eight digits at once (to skip constant-digit passes).
-__global__ __launch_bounds__(256) void k_rs_hist8(const uint64_t* __restrict__ keys, size_t n,
+// Histograms of digits [dlo, dhi) at once (to skip constant-digit passes).
+__global__ __launch_bounds__(256) void k_rs_hist8(const uint64_t* __restrict__ keys, size_t n, int dlo, int dhi,
                                                    uint32_t* __restrict__ hist8 /*[8][256]*/) {
     __shared__ uint32_t h[8][RS_BINS];
     for (int k = threadIdx.x; k < 8 * RS_BINS; k += blockDim.x) (&h[0][0])[k] = 0;
@@ -217,8 +217,7 @@ __global__ __launch_bounds__(256) void k_rs_hist8(const uint64_t* __restrict__ k
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
         uint64_t key = keys[i];
-#pragma unroll
-        for (int d = 0; d < 8; ++d) atomicAdd(&h[d][(key >> (8 * d)) & 255], 1u);
+        for (int d = dlo; d < dhi; ++d) atomicAdd(&h[d][(key >> (8 * d)) & 255], 1u);
     }
     __syncthreads();
     for (int k = threadIdx.x; k < 8 * RS_BINS; k += blockDim.x) {
@@ -368,7 +367,7 @@ int radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, siz
         k_zero_u32<<<grid_for(8 * RS_BINS, 256), 256, 0, st>>>(tmp.hist8, 8 * RS_BINS);
         LAUNCH_CHECK();
         unsigned hb = std::min<unsigned>(grid_for(n, 256 * 16), 2048);
-        k_rs_hist8<<<hb, 256, 0, st>>>(k0, n, tmp.hist8);
+        k_rs_hist8<<<hb, 256, 0, st>>>(k0, n, digit_lo, digit_hi, tmp.hist8);
         LAUNCH_CHECK();
         HIP_TRY(hipMemcpyAsync(h8, tmp.hist8, sizeof(h8), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));

@@ -135,7 +135,7 @@ template <typename S>
 __global__ __launch_bounds__(Q_THREADS) void k_plan_finish(IndexView<S> ix, const uint64_t* __restrict__ key,
                                                            const uint32_t* __restrict__ val,
                                                            const S* __restrict__ qs, size_t nq,
-                                                           PlanView<S> pv) {
+                                                           PlanView<S> pv, bool with_ub) {
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) pv.counter[0] = 0;
     if (k >= nq) return;
@@ -154,8 +154,12 @@ __global__ __launch_bounds__(Q_THREADS) void k_plan_finish(IndexView<S> ix, cons
     pv.q[k] = q;
     pv.s[k] = s;
     pv.e[k] = e;
-    pv.idx[k] = upper_bound_pos(ix.starts, ix.n, e);
+    if (with_ub) pv.idx[k] = upper_bound_pos(ix.starts, ix.n, e);
 }
+
+// upper_bound of n keys through the index's search tree (defined with its kernel below).
+template <typename S>
+int launch_ub_tree(const siIndex* ix, const S* xs, size_t n, uint32_t* out, hipStream_t st);
 
 // Number of sorted ends below x (branchless lower_bound).
 template <typename S>
@@ -242,8 +246,12 @@ int plan_impl(const siIndex* ix, const S* qs, const S* qe, size_t nq, void* plan
         rc = radix_sort_pairs(k0, v0, k1, v1, nq, 0, 8, rt, &ks, &vs, st, g_tune.plan_sync != 0);
     }
     if (rc != SI_OK) return rc;
-    k_plan_finish<S><<<g, Q_THREADS, 0, st>>>(ix->view<S>(), ks, vs, qs, nq, plan_view<S>(plan, nq));
+    // upper_bound of the sorted ends: through the search tree when the index has one
+    const bool tree = g_tune.ub_tree && ix->d_top && ix->n > 0;
+    const PlanView<S> pv = plan_view<S>(plan, nq);
+    k_plan_finish<S><<<g, Q_THREADS, 0, st>>>(ix->view<S>(), ks, vs, qs, nq, pv, !tree);
     LAUNCH_CHECK();
+    if (tree) return launch_ub_tree<S>(ix, pv.e, nq, pv.idx, st);
     return SI_OK;
 }
 
@@ -787,6 +795,30 @@ __global__ __launch_bounds__(Q_THREADS) void k_upper_bound_tree(UbTree<S> t, con
 }
 
 template <typename S>
+int launch_ub_tree(const siIndex* ix, const S* xs, size_t n, uint32_t* out, hipStream_t st) {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        cus = std::max(cus, 1);
+    }
+    const unsigned g = (unsigned)std::min<size_t>(grid_for(n, Q_THREADS), (size_t)cus * 8);
+    const size_t lds = (size_t)((1u << ix->top_h) - 1u) * ix->cb;
+    UbTree<S> t;
+    t.nlv = ix->nlv;
+    t.lv[0] = (const S*)ix->d_starts;
+    t.size[0] = (uint32_t)ix->n;
+    for (int l = 1; l < MAX_LEVELS; ++l) {
+        t.lv[l] = (l <= ix->nlv) ? (const S*)ix->d_lv + ix->lv_off[l] : nullptr;
+        t.size[l] = (l <= ix->nlv) ? ix->lv_size[l] : 0u;
+    }
+    k_upper_bound_tree<S><<<g, Q_THREADS, lds, st>>>(t, (const S*)ix->d_top, ix->top_h, xs, n, out);
+    LAUNCH_CHECK();
+    return SI_OK;
+}
+
+template <typename S>
 __global__ __launch_bounds__(Q_THREADS) void k_has_overlaps(IndexView<S> ix, const S* __restrict__ qs,
                                                             const S* __restrict__ qe, size_t nq,
                                                             uint8_t* __restrict__ out) {
@@ -1169,30 +1201,8 @@ SI_EXPORT int siUpperBound(const siIndex* ix, const void* xs, size_t n, uint32_t
     if (n == 0) return SI_OK;
     if (!xs || !out) return si_set_error(SI_ERR_ARG, "null buffer");
     hipStream_t st = (hipStream_t)stream;
-    if (g_tune.ub_tree && ix->d_top && ix->n > 0) {
-        static int cus = 0;
-        if (cus == 0) {
-            int dev = 0;
-            hipGetDevice(&dev);
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-            cus = std::max(cus, 1);
-        }
-        const unsigned g = (unsigned)std::min<size_t>(grid_for(n, Q_THREADS), (size_t)cus * 8);
-        const size_t lds = (size_t)((1u << ix->top_h) - 1u) * ix->cb;
-        return DISPATCH_S(ix, ([&]() -> int {
-            UbTree<S> t;
-            t.nlv = ix->nlv;
-            t.lv[0] = (const S*)ix->d_starts;
-            t.size[0] = (uint32_t)ix->n;
-            for (int l = 1; l < MAX_LEVELS; ++l) {
-                t.lv[l] = (l <= ix->nlv) ? (const S*)ix->d_lv + ix->lv_off[l] : nullptr;
-                t.size[l] = (l <= ix->nlv) ? ix->lv_size[l] : 0u;
-            }
-            k_upper_bound_tree<S><<<g, Q_THREADS, lds, st>>>(t, (const S*)ix->d_top, ix->top_h, (const S*)xs, n, out);
-            LAUNCH_CHECK();
-            return SI_OK;
-        }()));
-    }
+    if (g_tune.ub_tree && ix->d_top && ix->n > 0)
+        return DISPATCH_S(ix, (launch_ub_tree<S>(ix, (const S*)xs, n, out, st)));
     return DISPATCH_S(ix, ([&]() -> int {
         k_upper_bound<S><<<grid_for(n, Q_THREADS), Q_THREADS, 0, st>>>(ix->view<S>(), (const S*)xs, n, out);
         LAUNCH_CHECK();
