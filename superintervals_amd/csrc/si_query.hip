@@ -1077,12 +1077,18 @@ SI_EXPORT int siSearchOffsetsPlanned(const siIndex* ix, void* plan, size_t nq, u
     if (rc) return rc;
     if (!plan || !offsets) return si_set_error(SI_ERR_ARG, "null plan/offsets");
     hipStream_t st = (hipStream_t)stream;
-    if (g_tune.offsets_rank)
-        rc = DISPATCH_S(ix, (launch_rank<S>(ix, plan, nq, counts, offsets, st)));
-    else
-        rc = DISPATCH_S(ix, (launch_walk<S, W_COUNT>(ix, plan, nq, counts, offsets, nullptr, nullptr, nullptr, st)));
+    const PlanLayout L(ix->cb, nq);
+    uint64_t* ws = (uint64_t*)((char*)plan + L.off_scratch);
+    if (g_tune.offsets_rank) {
+        // 32-bit counts (the caller's, or the heavy-list area of the plan, which the rank
+        // form does not use), then one scan to 64-bit offsets: half the scattered bytes
+        uint32_t* c32 = counts ? counts : (uint32_t*)((char*)plan + L.off_heavy);
+        rc = DISPATCH_S(ix, (launch_rank<S>(ix, plan, nq, c32, nullptr, st)));
+        if (rc) return rc;
+        return device_excl_scan<uint32_t, uint64_t>(c32, nq, offsets, offsets + nq, ws, st);
+    }
+    rc = DISPATCH_S(ix, (launch_walk<S, W_COUNT>(ix, plan, nq, counts, offsets, nullptr, nullptr, nullptr, st)));
     if (rc) return rc;
-    uint64_t* ws = (uint64_t*)((char*)plan + PlanLayout(ix->cb, nq).off_scratch);
     return device_excl_scan<uint64_t, uint64_t>(offsets, nq, offsets, offsets + nq, ws, st);
 }
 
