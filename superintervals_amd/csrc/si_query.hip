@@ -327,44 +327,50 @@ __device__ __forceinline__ uint32_t kbit_entry(uint32_t b) {
     return e;
 }
 
+// Segment descriptor written at its start slot of the current chunk.
+__device__ __forceinline__ void put_segment(EmitSlot* __restrict__ sd, uint32_t slot, uint32_t pre, uint32_t c,
+                                            uint32_t m, uint32_t base, bool asc, uint64_t o) {
+    EmitSlot e;
+    e.pre = pre;
+    e.kv = m | ((asc ? 1u : 0u) << 16) | 0x80000000u;
+    e.base = base;
+    e.c = c;
+    e.o_lo = (uint32_t)o;
+    e.o_hi = (uint32_t)(o >> 32);
+    sd[slot] = e;
+}
+
+// Every lane holds up to two segments (the walk emits every other step: the previous
+// step's segment and this one's); they occupy consecutive slots of the concatenation.
 template <typename S, int MODE>
-__device__ __forceinline__ void emit_segments(uint32_t c, uint32_t m, uint32_t base, bool asc,
-                                              uint64_t o, int lane, EmitSlot* __restrict__ sd,
+__device__ __forceinline__ void emit_segments(uint32_t c1, uint32_t m1, uint32_t base1, bool asc1, uint64_t o1,
+                                              uint32_t c2, uint32_t m2, uint32_t base2, bool asc2, uint64_t o2,
+                                              int lane, EmitSlot* __restrict__ sd,
                                               const uint32_t* __restrict__ kbit,
                                               uint32_t* __restrict__ out_idx,
                                               const int32_t* __restrict__ vals,
                                               int32_t* __restrict__ out_val) {
     constexpr int W = Line<S>::W;
+    const uint32_t c = c1 + c2;
     if (__ballot(c != 0) == 0) return;
     const uint32_t inc = wave_incl_scan_dpp(c);
     const uint32_t total = __builtin_amdgcn_readlane(inc, WAVE - 1);
-    const uint32_t pre = inc - c;
+    const uint32_t pre1 = inc - c, pre2 = pre1 + c1;
     const uint64_t le = (lane == WAVE - 1) ? ~0ull : ((2ull << lane) - 1ull);   // lanes <= this
-    EmitSlot cy = {0u, 0u, 0u, 0u, 0u, 0u};
     for (uint32_t x0 = 0; x0 < total; x0 += WAVE) {
-        if (c != 0 && pre >= x0 && pre - x0 < (uint32_t)WAVE) {
-            EmitSlot e;
-            e.pre = pre;
-            e.kv = m | ((asc ? 1u : 0u) << 16) | 0x80000000u;
-            e.base = base;
-            e.c = c;
-            e.o_lo = (uint32_t)o;
-            e.o_hi = (uint32_t)(o >> 32);
-            sd[pre - x0] = e;
-        }
+        if (c1 != 0 && pre1 >= x0 && pre1 - x0 < (uint32_t)WAVE)
+            put_segment(sd, pre1 - x0, pre1, c1, m1, base1, asc1, o1);
+        if (c2 != 0 && pre2 >= x0 && pre2 - x0 < (uint32_t)WAVE)
+            put_segment(sd, pre2 - x0, pre2, c2, m2, base2, asc2, o2);
         wave_lds_sync();
         const uint64_t mine = __ballot((sd[lane].kv >> 31) != 0u) & le;
-        EmitSlot d = cy;
-        if (mine) d = sd[63u - (uint32_t)__builtin_clzll(mine)];
+        // slot WAVE holds the segment running in from the previous chunk (never read in
+        // the first chunk, whose slot 0 always starts a segment)
+        const EmitSlot d = sd[mine ? 63u - (uint32_t)__builtin_clzll(mine) : (uint32_t)WAVE];
         wave_lds_sync();
         sd[lane].kv = 0u;   // free the slot for the next chunk (every read above is done)
         // the segment holding the chunk's last slot runs into the next chunk
-        cy.pre = __builtin_amdgcn_readlane(d.pre, WAVE - 1);
-        cy.kv = __builtin_amdgcn_readlane(d.kv, WAVE - 1);
-        cy.base = __builtin_amdgcn_readlane(d.base, WAVE - 1);
-        cy.c = __builtin_amdgcn_readlane(d.c, WAVE - 1);
-        cy.o_lo = __builtin_amdgcn_readlane(d.o_lo, WAVE - 1);
-        cy.o_hi = __builtin_amdgcn_readlane(d.o_hi, WAVE - 1);
+        if (lane == WAVE - 1) sd[WAVE] = d;
         const uint32_t x = x0 + (uint32_t)lane;
         const uint32_t r = x - d.pre;
         const uint32_t om = d.kv & 0xFFFFu;
@@ -412,7 +418,7 @@ __device__ __forceinline__ void emit_segments(uint32_t c, uint32_t m, uint32_t b
 #define SI_WALK_WIN32 3328
 #endif
 #ifndef SI_WALK_WIN64
-#define SI_WALK_WIN64 2288
+#define SI_WALK_WIN64 2272
 #endif
 #ifndef SI_WALK_WAVES_PER_EU
 #define SI_WALK_WAVES_PER_EU 8
@@ -440,11 +446,17 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
     __shared__ __align__(16) S s_end[WIN];
     __shared__ __align__(16) uint32_t s_ex[WIN];   // exit targets (IndexView::exitp)
     __shared__ uint32_t s_tile, s_next;
-    __shared__ EmitSlot s_emit[(MODE == W_COUNT) ? 1 : T_THREADS];   // per-wave emission slots
+    // per-wave emission slots: 64 + one for the segment carried between chunks
+    __shared__ EmitSlot s_emit[(MODE == W_COUNT) ? 1 : T_THREADS / WAVE * (WAVE + 1)];
     __shared__ uint32_t s_kbit[(MODE == W_COUNT || !SI_EMIT_LUT) ? 1 : 256];   // byte set-bit table
     const int lane = lane_id();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    if (MODE != W_COUNT) s_emit[threadIdx.x].kv = 0u;   // emission slots start free (own wave's)
+    EmitSlot* const my_emit = s_emit + (MODE == W_COUNT ? 0u : (threadIdx.x / WAVE) * (WAVE + 1));
+    if (MODE != W_COUNT) my_emit[lane].kv = 0u;   // emission slots start free (own wave's)
+    // segment held from the previous step (emission runs every other step)
+    uint32_t p_c = 0, p_m = 0, p_base = 0;
+    bool p_asc = false, hold = true;
+    uint64_t p_o = 0;
     if (MODE != W_COUNT && SI_EMIT_LUT) {
         for (uint32_t b = threadIdx.x; b < 256u; b += T_THREADS) s_kbit[b] = kbit_entry(b);
     }   // read only after the first tile's __syncthreads
@@ -573,8 +585,18 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
                 }
             }
             if (MODE != W_COUNT) {
-                emit_segments<S, MODE>(c_emit, e_m, e_base, e_asc, e_o, lane, s_emit + (threadIdx.x & ~63u), s_kbit,
-                                       out_idx, vals, out_val);
+                if (hold) {
+                    p_c = c_emit;
+                    p_m = e_m;
+                    p_base = e_base;
+                    p_asc = e_asc;
+                    p_o = e_o;
+                } else {
+                    emit_segments<S, MODE>(p_c, p_m, p_base, p_asc, p_o, c_emit, e_m, e_base, e_asc, e_o, lane,
+                                           my_emit, s_kbit, out_idx, vals, out_val);
+                    p_c = 0;
+                }
+                hold = !hold;   // wave-uniform: every lane runs every step
             }
             if (has && cur == NONE) {
                 if (MODE == W_COUNT) {
@@ -599,6 +621,10 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
         }
         __syncthreads();   // LDS window and counters are reused by the next tile
     }
+    // the segments held from the last step (all lanes of the workgroup left the tile loop)
+    if (MODE != W_COUNT)
+        emit_segments<S, MODE>(p_c, p_m, p_base, p_asc, p_o, 0u, 0u, 0u, false, 0ull, lane, my_emit, s_kbit,
+                               out_idx, vals, out_val);
 }
 
 // Wave-cooperative continuation of parked walks: one wave per query, one aligned line of
