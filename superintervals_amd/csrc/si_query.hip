@@ -329,10 +329,10 @@ __device__ __forceinline__ uint32_t kbit_entry(uint32_t b) {
 
 // Segment descriptor written at its start slot of the current chunk.
 __device__ __forceinline__ void put_segment(EmitSlot* __restrict__ sd, uint32_t slot, uint32_t pre, uint32_t c,
-                                            uint32_t m, uint32_t base, bool asc, uint64_t o) {
+                                            uint32_t mk, uint32_t base, uint64_t o) {
     EmitSlot e;
     e.pre = pre;
-    e.kv = m | ((asc ? 1u : 0u) << 16) | 0x80000000u;
+    e.kv = mk | 0x80000000u;
     e.base = base;
     e.c = c;
     e.o_lo = (uint32_t)o;
@@ -340,28 +340,45 @@ __device__ __forceinline__ void put_segment(EmitSlot* __restrict__ sd, uint32_t 
     sd[slot] = e;
 }
 
-// Every lane holds up to two segments (the walk emits every other step: the previous
-// step's segment and this one's); they occupy consecutive slots of the concatenation.
-template <typename S, int MODE>
-__device__ __forceinline__ void emit_segments(uint32_t c1, uint32_t m1, uint32_t base1, bool asc1, uint64_t o1,
-                                              uint32_t c2, uint32_t m2, uint32_t base2, bool asc2, uint64_t o2,
-                                              int lane, EmitSlot* __restrict__ sd,
+// One output segment of a lane: c values at output offset o, either the ascending range
+// base .. base + c - 1 (asc) or the set bits of m (descending) offset by base.
+struct Seg {
+    uint32_t c, mk, base;   // mk = m | asc << 16
+    uint64_t o;
+};
+
+// Steps whose segments the fill walk holds and emits together (their segments of one
+// query are adjacent in its output list, so each emission writes longer runs).
+#ifndef SI_EMIT_EVERY
+#define SI_EMIT_EVERY 2
+#endif
+constexpr int EMIT_EVERY = SI_EMIT_EVERY;
+static_assert(EMIT_EVERY >= 2 && EMIT_EVERY <= 4, "2 to 4 segments per emission");
+
+// Every lane holds up to N segments; they occupy consecutive slots of the concatenation.
+template <typename S, int MODE, int N>
+__device__ __forceinline__ void emit_segments(const Seg (&sg)[N], int lane, EmitSlot* __restrict__ sd,
                                               const uint32_t* __restrict__ kbit,
                                               uint32_t* __restrict__ out_idx,
                                               const int32_t* __restrict__ vals,
                                               int32_t* __restrict__ out_val) {
     constexpr int W = Line<S>::W;
-    const uint32_t c = c1 + c2;
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) c += sg[j].c;
     if (__ballot(c != 0) == 0) return;
     const uint32_t inc = wave_incl_scan_dpp(c);
     const uint32_t total = __builtin_amdgcn_readlane(inc, WAVE - 1);
-    const uint32_t pre1 = inc - c, pre2 = pre1 + c1;
+    const uint32_t pre0 = inc - c;
     const uint64_t le = (lane == WAVE - 1) ? ~0ull : ((2ull << lane) - 1ull);   // lanes <= this
     for (uint32_t x0 = 0; x0 < total; x0 += WAVE) {
-        if (c1 != 0 && pre1 >= x0 && pre1 - x0 < (uint32_t)WAVE)
-            put_segment(sd, pre1 - x0, pre1, c1, m1, base1, asc1, o1);
-        if (c2 != 0 && pre2 >= x0 && pre2 - x0 < (uint32_t)WAVE)
-            put_segment(sd, pre2 - x0, pre2, c2, m2, base2, asc2, o2);
+        uint32_t pre = pre0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            if (sg[j].c != 0 && pre >= x0 && pre - x0 < (uint32_t)WAVE)
+                put_segment(sd, pre - x0, pre, sg[j].c, sg[j].mk, sg[j].base, sg[j].o);
+            pre += sg[j].c;
+        }
         wave_lds_sync();
         const uint64_t mine = __ballot((sd[lane].kv >> 31) != 0u) & le;
         // slot WAVE holds the segment running in from the previous chunk (never read in
@@ -453,10 +470,11 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     EmitSlot* const my_emit = s_emit + (MODE == W_COUNT ? 0u : (threadIdx.x / WAVE) * (WAVE + 1));
     if (MODE != W_COUNT) my_emit[lane].kv = 0u;   // emission slots start free (own wave's)
-    // segment held from the previous step (emission runs every other step)
-    uint32_t p_c = 0, p_m = 0, p_base = 0;
-    bool p_asc = false, hold = true;
-    uint64_t p_o = 0;
+    // segments held from the previous steps (emission runs every EMIT_EVERY steps)
+    Seg held[EMIT_EVERY - 1];
+#pragma unroll
+    for (int j = 0; j < EMIT_EVERY - 1; ++j) held[j] = Seg{0u, 0u, 0u, 0ull};
+    int phase = 0;
     if (MODE != W_COUNT && SI_EMIT_LUT) {
         for (uint32_t b = threadIdx.x; b < 256u; b += T_THREADS) s_kbit[b] = kbit_entry(b);
     }   // read only after the first tile's __syncthreads
@@ -585,18 +603,25 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
                 }
             }
             if (MODE != W_COUNT) {
-                if (hold) {
-                    p_c = c_emit;
-                    p_m = e_m;
-                    p_base = e_base;
-                    p_asc = e_asc;
-                    p_o = e_o;
+                // hold this step's segment; every EMIT_EVERY steps emit the held ones with
+                // it (phase is wave-uniform: every lane runs every step)
+                const Seg now = Seg{c_emit, e_m | ((e_asc ? 1u : 0u) << 16), e_base, e_o};
+                if (phase == EMIT_EVERY - 1) {
+                    Seg all[EMIT_EVERY];
+#pragma unroll
+                    for (int j = 0; j < EMIT_EVERY - 1; ++j) all[j] = held[j];
+                    all[EMIT_EVERY - 1] = now;
+                    emit_segments<S, MODE, EMIT_EVERY>(all, lane, my_emit, s_kbit, out_idx, vals, out_val);
+#pragma unroll
+                    for (int j = 0; j < EMIT_EVERY - 1; ++j) held[j].c = 0;
+                    phase = 0;
                 } else {
-                    emit_segments<S, MODE>(p_c, p_m, p_base, p_asc, p_o, c_emit, e_m, e_base, e_asc, e_o, lane,
-                                           my_emit, s_kbit, out_idx, vals, out_val);
-                    p_c = 0;
+#pragma unroll
+                    for (int j = 0; j < EMIT_EVERY - 1; ++j) {
+                        if (phase == j) held[j] = now;
+                    }
+                    ++phase;
                 }
-                hold = !hold;   // wave-uniform: every lane runs every step
             }
             if (has && cur == NONE) {
                 if (MODE == W_COUNT) {
@@ -621,10 +646,8 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
         }
         __syncthreads();   // LDS window and counters are reused by the next tile
     }
-    // the segments held from the last step (all lanes of the workgroup left the tile loop)
-    if (MODE != W_COUNT)
-        emit_segments<S, MODE>(p_c, p_m, p_base, p_asc, p_o, 0u, 0u, 0u, false, 0ull, lane, my_emit, s_kbit,
-                               out_idx, vals, out_val);
+    // the segments held from the last steps (all lanes of the workgroup left the tile loop)
+    if (MODE != W_COUNT) emit_segments<S, MODE, EMIT_EVERY - 1>(held, lane, my_emit, s_kbit, out_idx, vals, out_val);
 }
 
 // Wave-cooperative continuation of parked walks: one wave per query, one aligned line of
