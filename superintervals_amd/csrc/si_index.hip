@@ -227,6 +227,40 @@ __global__ __launch_bounds__(256) void k_branch(TreeDesc<S> td, uint32_t* __rest
 
 namespace {
 
+// Search tree over n sorted keys: 64-ary levels of first keys up to a top level of at
+// most 4095 keys, which is also stored in Eytzinger order (k_every64, k_eytz).
+template <typename S>
+int build_key_tree(const S* keys, size_t n, KeyTree& t, hipStream_t st) {
+    uint32_t sz[MAX_LEVELS + 1];
+    sz[0] = (uint32_t)n;
+    int nl = 0;
+    size_t tot = 0;
+    while (sz[nl] > 4095 && nl < MAX_LEVELS - 1) {
+        sz[nl + 1] = (uint32_t)((sz[nl] + 63) / 64);
+        ++nl;
+        t.lv_off[nl] = tot;
+        t.lv_size[nl] = sz[nl];
+        tot += sz[nl];
+    }
+    t.nlv = nl;
+    if (tot) HIP_TRY(hipMalloc(&t.d_lv, tot * sizeof(S)));
+    for (int l = 1; l <= nl; ++l) {
+        const S* in = (l == 1) ? keys : (const S*)t.d_lv + t.lv_off[l - 1];
+        k_every64<S><<<grid_for(sz[l], 256), 256, 0, st>>>(in, sz[l], (S*)t.d_lv + t.lv_off[l]);
+        LAUNCH_CHECK();
+    }
+    const uint32_t m = sz[nl];
+    uint32_t h = 0;
+    while (((1u << h) - 1u) < m) ++h;
+    t.top_size = m;
+    t.top_h = h;
+    HIP_TRY(hipMalloc(&t.d_top, (size_t)((1u << h) - 1u) * sizeof(S)));
+    const S* topl = (nl == 0) ? keys : (const S*)t.d_lv + t.lv_off[nl];
+    k_eytz<S><<<grid_for((1u << h) - 1u, 256), 256, 0, st>>>(topl, m, h, (S*)t.d_top);
+    LAUNCH_CHECK();
+    return SI_OK;
+}
+
 template <typename S>
 int build_impl(siIndex* ix, const void* starts, const void* ends, const int32_t* values, size_t n,
                int start_sorted, int end_sorted, int on_device, hipStream_t st) {
@@ -346,35 +380,10 @@ int build_impl(siIndex* ix, const void* starts, const void* ends, const int32_t*
     k_exit<S><<<grid_for(npad, 256), 256, 0, st>>>((const S*)ix->d_ends, ix->d_branch, (uint32_t)n, (uint32_t)npad,
                                                  ix->d_exit);
     LAUNCH_CHECK();
-    // upper_bound search tree: 64-ary levels of first keys, top level (<= 4096) in Eytzinger order
+    // upper_bound search tree over the starts
     {
-        uint32_t sz[MAX_LEVELS + 1];
-        sz[0] = (uint32_t)n;
-        int nl = 0;
-        size_t tot = 0;
-        while (sz[nl] > 4095 && nl < MAX_LEVELS - 1) {
-            sz[nl + 1] = (uint32_t)((sz[nl] + 63) / 64);
-            ++nl;
-            ix->lv_off[nl] = tot;
-            ix->lv_size[nl] = sz[nl];
-            tot += sz[nl];
-        }
-        ix->nlv = nl;
-        if (tot) HIP_TRY(hipMalloc(&ix->d_lv, tot * sizeof(S)));
-        for (int l = 1; l <= nl; ++l) {
-            const S* in = (l == 1) ? (const S*)ix->d_starts : (const S*)ix->d_lv + ix->lv_off[l - 1];
-            k_every64<S><<<grid_for(sz[l], 256), 256, 0, st>>>(in, sz[l], (S*)ix->d_lv + ix->lv_off[l]);
-            LAUNCH_CHECK();
-        }
-        const uint32_t m = sz[nl];
-        uint32_t h = 0;
-        while (((1u << h) - 1u) < m) ++h;
-        ix->top_size = m;
-        ix->top_h = h;
-        HIP_TRY(hipMalloc(&ix->d_top, (size_t)((1u << h) - 1u) * sizeof(S)));
-        const S* topl = (nl == 0) ? (const S*)ix->d_starts : (const S*)ix->d_lv + ix->lv_off[nl];
-        k_eytz<S><<<grid_for((1u << h) - 1u, 256), 256, 0, st>>>(topl, m, h, (S*)ix->d_top);
-        LAUNCH_CHECK();
+        const int rc = build_key_tree<S>((const S*)ix->d_starts, n, ix->ub, st);
+        if (rc != SI_OK) return rc;
     }
     // ends in ascending order, for the rank form of count
     {
@@ -403,6 +412,11 @@ int build_impl(siIndex* ix, const void* starts, const void* ends, const int32_t*
         hipFree(tmp);
         if (rc != SI_OK) return rc;
         if (es != hipSuccess) return si_set_error(SI_ERR_HIP, "build: %s", hipGetErrorString(es));
+    }
+    // search tree over the sorted ends (rank count)
+    {
+        const int rc = build_key_tree<S>((const S*)ix->d_ends_sorted, n, ix->eb, st);
+        if (rc != SI_OK) return rc;
     }
     HIP_TRY(hipStreamSynchronize(st));
     if (!on_device) { hipFree(in_s); hipFree(in_e); if (in_v) hipFree(in_v); }

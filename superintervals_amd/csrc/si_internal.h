@@ -8,6 +8,25 @@
 
 using namespace sik;
 
+// Search tree over a sorted key array: level l + 1 holds every 64th key of level l
+// (level 0 = the keys themselves) up to a top level of at most 4095 keys, which is also
+// kept in Eytzinger (breadth-first) order with max(S) padding to 2^top_h - 1 keys.
+struct KeyTree {
+    void* d_lv = nullptr;            // levels 1 .. nlv, contiguous
+    void* d_top = nullptr;
+    int nlv = 0;                     // levels above the keys
+    size_t lv_off[MAX_LEVELS] = {0}; // element offsets of levels 1 .. nlv in d_lv
+    uint32_t lv_size[MAX_LEVELS] = {0};
+    uint32_t top_size = 0, top_h = 0;
+
+    void release() {
+        hipFree(d_lv); hipFree(d_top);
+        d_lv = d_top = nullptr;
+        nlv = 0;
+        top_size = top_h = 0;
+    }
+};
+
 // ============================================================================
 // siIndex
 // ============================================================================
@@ -24,15 +43,8 @@ struct siIndex {
     int32_t* d_values = nullptr;
     void* d_tree = nullptr;   // levels 1.. of the max tree, contiguous
     void* d_ends_sorted = nullptr;   // ends in ascending order (rank count)
-    // upper_bound search tree: level l + 1 holds every 64th key of level l (level 0 =
-    // starts) up to a top level of at most 4096 keys, which is also kept in Eytzinger
-    // (breadth-first) order with max(S) padding to 2^top_h - 1 keys
-    void* d_lv = nullptr;            // levels 1 .. nlv, contiguous
-    void* d_top = nullptr;
-    int nlv = 0;                     // levels above the starts
-    size_t lv_off[MAX_LEVELS] = {0}; // element offsets of levels 1 .. nlv in d_lv
-    uint32_t lv_size[MAX_LEVELS] = {0};
-    uint32_t top_size = 0, top_h = 0;
+    KeyTree ub;   // search tree over the starts (upper_bound)
+    KeyTree eb;   // search tree over the sorted ends (rank count)
     int nlev = 0;
     size_t lvl_off[MAX_LEVELS] = {0};   // element offsets into d_tree (level >= 1)
     uint32_t lvl_size[MAX_LEVELS] = {0};
@@ -40,10 +52,10 @@ struct siIndex {
 
     void release() {
         hipFree(d_starts); hipFree(d_ends); hipFree(d_branch); hipFree(d_exit); hipFree(d_perm);
-        hipFree(d_values); hipFree(d_tree); hipFree(d_ends_sorted); hipFree(d_lv); hipFree(d_top);
-        d_starts = d_ends = d_tree = d_ends_sorted = d_lv = d_top = nullptr;
-        nlv = 0;
-        top_size = top_h = 0;
+        hipFree(d_values); hipFree(d_tree); hipFree(d_ends_sorted);
+        d_starts = d_ends = d_tree = d_ends_sorted = nullptr;
+        ub.release();
+        eb.release();
         d_branch = d_exit = d_perm = nullptr;
         d_values = nullptr;
         n = 0;

@@ -157,7 +157,8 @@ __global__ __launch_bounds__(Q_THREADS) void k_plan_finish(IndexView<S> ix, cons
     if (with_ub) pv.idx[k] = upper_bound_pos(ix.starts, ix.n, e);
 }
 
-// upper_bound of n keys through the index's search tree (defined with its kernel below).
+// upper_bound of n keys through the index's search tree over the starts (defined with its
+// kernel below).
 template <typename S>
 int launch_ub_tree(const siIndex* ix, const S* xs, size_t n, uint32_t* out, hipStream_t st);
 
@@ -247,7 +248,7 @@ int plan_impl(const siIndex* ix, const S* qs, const S* qe, size_t nq, void* plan
     }
     if (rc != SI_OK) return rc;
     // upper_bound of the sorted ends: through the search tree when the index has one
-    const bool tree = g_tune.ub_tree && ix->d_top && ix->n > 0;
+    const bool tree = g_tune.ub_tree && ix->ub.d_top && ix->n > 0;
     const PlanView<S> pv = plan_view<S>(plan, nq);
     k_plan_finish<S><<<g, Q_THREADS, 0, st>>>(ix->view<S>(), ks, vs, qs, nq, pv, !tree);
     LAUNCH_CHECK();
@@ -809,6 +810,24 @@ struct UbTree {
     int nlv;                     // levels above the starts
 };
 
+// Last position with key <= x, or NONE, through the tree (s_top = its top level in LDS).
+template <typename S>
+__device__ __forceinline__ uint32_t ub_tree_pos(const UbTree<S>& t, const S* s_top, uint32_t h, S x) {
+    uint32_t k = 1, r = 0;
+    for (uint32_t lvl = 0; lvl < h; ++lvl) {
+        const bool right = s_top[k - 1] <= x;
+        r += right ? (1u << (h - 1u - lvl)) : 0u;   // the left subtree and the node itself
+        k = 2u * k + (right ? 1u : 0u);
+    }
+    // padding keys are max(S); a key equal to max(S) may count padding, so clamp
+    int64_t j = (int64_t)std::min<uint32_t>(r, t.size[t.nlv]) - 1;
+    for (int l = t.nlv - 1; l >= 0 && j >= 0; --l) {
+        const int64_t b = j * 64, e = std::min<int64_t>(b + 63, (int64_t)t.size[l] - 1);
+        j = last_le_in(t.lv[l], b, e, x);
+    }
+    return (j < 0) ? NONE : (uint32_t)j;
+}
+
 // upper_bound through the search tree: descend the Eytzinger top (LDS) to r = the number of
 // top-level keys <= x, then at each level below search the one 64-key node under the entry
 // found (its first key is that entry, so it is <= x and the node holds the answer for the
@@ -826,25 +845,60 @@ __global__ __launch_bounds__(Q_THREADS) void k_upper_bound_tree(UbTree<S> t, con
     for (size_t q0 = (size_t)blockIdx.x * Q_THREADS; q0 < nx; q0 += (size_t)gridDim.x * Q_THREADS) {
         const size_t q = q0 + threadIdx.x;
         if (q >= nx) continue;
-        const S x = xs[q];
-        uint32_t k = 1, r = 0;
-        for (uint32_t lvl = 0; lvl < h; ++lvl) {
-            const bool right = s_top[k - 1] <= x;
-            r += right ? (1u << (h - 1u - lvl)) : 0u;   // the left subtree and the node itself
-            k = 2u * k + (right ? 1u : 0u);
+        out[q] = ub_tree_pos(t, s_top, h, xs[q]);
+    }
+}
+
+// Rank form of count for the planned queries (as k_count_rank) with #{ends < s} found
+// through the search tree over the sorted ends: #{ends < s} = #{ends <= s - 1} =
+// upper_bound_pos(s - 1) + 1 (NONE + 1 wraps to 0), and 0 when s is the smallest value.
+template <typename S>
+__global__ __launch_bounds__(Q_THREADS) void k_count_rank_tree(IndexView<S> ix, UbTree<S> t,
+                                                               const S* __restrict__ top, uint32_t h,
+                                                               PlanView<S> pv, size_t nq,
+                                                               uint32_t* __restrict__ counts,
+                                                               uint64_t* __restrict__ counts64) {
+    extern __shared__ __align__(16) char s_mem[];
+    S* s_top = reinterpret_cast<S*>(s_mem);
+    const uint32_t T = (1u << h) - 1u;
+    for (uint32_t k = threadIdx.x; k < T; k += Q_THREADS) s_top[k] = top[k];
+    __syncthreads();
+    constexpr S SMIN = (sizeof(S) == 4) ? (S)INT32_MIN : (S)INT64_MIN;
+    for (size_t k0 = (size_t)blockIdx.x * Q_THREADS; k0 < nq; k0 += (size_t)gridDim.x * Q_THREADS) {
+        const size_t k = k0 + threadIdx.x;
+        if (k >= nq) continue;
+        const uint32_t i = pv.idx[k];
+        const S s = pv.s[k], e = pv.e[k];
+        uint32_t c = 0;
+        if (i != NONE) {
+            if (s <= e) {
+                const uint32_t below = (s == SMIN) ? 0u : ub_tree_pos(t, s_top, h, (S)(s - 1)) + 1u;
+                c = i + 1u - below;
+            } else {
+                c = walk_count(ix, i, s);
+            }
         }
-        // padding keys are max(S); a key equal to max(S) may count padding, so clamp
-        int64_t j = (int64_t)std::min<uint32_t>(r, t.size[t.nlv]) - 1;
-        for (int l = t.nlv - 1; l >= 0 && j >= 0; --l) {
-            const int64_t b = j * 64, e = std::min<int64_t>(b + 63, (int64_t)t.size[l] - 1);
-            j = last_le_in(t.lv[l], b, e, x);
-        }
-        out[q] = (j < 0) ? NONE : (uint32_t)j;
+        const uint32_t q = pv.q[k];
+        if (counts) counts[q] = c;
+        if (counts64) counts64[q] = c;
     }
 }
 
 template <typename S>
-int launch_ub_tree(const siIndex* ix, const S* xs, size_t n, uint32_t* out, hipStream_t st) {
+UbTree<S> ub_tree_of(const KeyTree& kt, const S* keys, size_t n) {
+    UbTree<S> t;
+    t.nlv = kt.nlv;
+    t.lv[0] = keys;
+    t.size[0] = (uint32_t)n;
+    for (int l = 1; l < MAX_LEVELS; ++l) {
+        t.lv[l] = (l <= kt.nlv) ? (const S*)kt.d_lv + kt.lv_off[l] : nullptr;
+        t.size[l] = (l <= kt.nlv) ? kt.lv_size[l] : 0u;
+    }
+    return t;
+}
+
+// Persistent grid of the tree kernels: enough blocks to fill the GPU, each stages the top once.
+inline unsigned tree_grid(size_t n) {
     static int cus = 0;
     if (cus == 0) {
         int dev = 0;
@@ -852,17 +906,15 @@ int launch_ub_tree(const siIndex* ix, const S* xs, size_t n, uint32_t* out, hipS
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         cus = std::max(cus, 1);
     }
-    const unsigned g = (unsigned)std::min<size_t>(grid_for(n, Q_THREADS), (size_t)cus * 8);
-    const size_t lds = (size_t)((1u << ix->top_h) - 1u) * ix->cb;
-    UbTree<S> t;
-    t.nlv = ix->nlv;
-    t.lv[0] = (const S*)ix->d_starts;
-    t.size[0] = (uint32_t)ix->n;
-    for (int l = 1; l < MAX_LEVELS; ++l) {
-        t.lv[l] = (l <= ix->nlv) ? (const S*)ix->d_lv + ix->lv_off[l] : nullptr;
-        t.size[l] = (l <= ix->nlv) ? ix->lv_size[l] : 0u;
-    }
-    k_upper_bound_tree<S><<<g, Q_THREADS, lds, st>>>(t, (const S*)ix->d_top, ix->top_h, xs, n, out);
+    return (unsigned)std::min<size_t>(grid_for(n, Q_THREADS), (size_t)cus * 8);
+}
+
+template <typename S>
+int launch_ub_tree(const siIndex* ix, const S* xs, size_t n, uint32_t* out, hipStream_t st) {
+    const KeyTree& kt = ix->ub;
+    const size_t lds = (size_t)((1u << kt.top_h) - 1u) * ix->cb;
+    k_upper_bound_tree<S><<<tree_grid(n), Q_THREADS, lds, st>>>(ub_tree_of<S>(kt, (const S*)ix->d_starts, ix->n),
+                                                                 (const S*)kt.d_top, kt.top_h, xs, n, out);
     LAUNCH_CHECK();
     return SI_OK;
 }
@@ -960,6 +1012,15 @@ template <typename S>
 int launch_rank(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint64_t* counts64,
                 hipStream_t st) {
     if (nq == 0) return SI_OK;
+    if (g_tune.ub_tree && ix->eb.d_top && ix->n > 0) {
+        const KeyTree& kt = ix->eb;
+        const size_t lds = (size_t)((1u << kt.top_h) - 1u) * ix->cb;
+        k_count_rank_tree<S><<<tree_grid(nq), Q_THREADS, lds, st>>>(
+            ix->view<S>(), ub_tree_of<S>(kt, (const S*)ix->d_ends_sorted, ix->n), (const S*)kt.d_top, kt.top_h,
+            plan_view<S>(plan, nq), nq, counts, counts64);
+        LAUNCH_CHECK();
+        return SI_OK;
+    }
     k_count_rank<S><<<grid_for(nq, Q_THREADS), Q_THREADS, 0, st>>>(
         ix->view<S>(), (const S*)ix->d_ends_sorted, plan_view<S>(plan, nq), nq, counts, counts64);
     LAUNCH_CHECK();
@@ -1256,7 +1317,7 @@ SI_EXPORT int siUpperBound(const siIndex* ix, const void* xs, size_t n, uint32_t
     if (n == 0) return SI_OK;
     if (!xs || !out) return si_set_error(SI_ERR_ARG, "null buffer");
     hipStream_t st = (hipStream_t)stream;
-    if (g_tune.ub_tree && ix->d_top && ix->n > 0)
+    if (g_tune.ub_tree && ix->ub.d_top && ix->n > 0)
         return DISPATCH_S(ix, (launch_ub_tree<S>(ix, (const S*)xs, n, out, st)));
     return DISPATCH_S(ix, ([&]() -> int {
         k_upper_bound<S><<<grid_for(n, Q_THREADS), Q_THREADS, 0, st>>>(ix->view<S>(), (const S*)xs, n, out);

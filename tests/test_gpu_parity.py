@@ -477,6 +477,29 @@ def test_empty_and_edge_cases(sia, orc):
     ixe = sia.DeviceIndex(np.int32).build(s, e)
     oix = orc.build_index(s, e)
     assert np.array_equal(ixe.count(qs, qe), orc.count_batch(oix, qs, qe))
+    # the rank form (#{ends < s} through the sorted-ends tree; s = min has none below) at
+    # both widths' extremes, for count and for the offsets of a search
+    from superintervals_amd import autotune
+    saved = autotune.current()
+    try:
+        for dt in (np.int32, np.int64):
+            lo, hi = np.iinfo(dt).min, np.iinfo(dt).max
+            s2 = np.array([lo, lo, 0, hi - 5, hi, lo + 1], dt)
+            e2 = np.array([lo, hi, 10, hi, hi, lo + 1], dt)
+            qs2 = np.array([lo, lo, 5, hi, hi, 0, hi, lo + 1, lo], dt)
+            qe2 = np.array([lo, hi, 5, hi, hi - 1, lo, lo, lo + 1, lo + 1], dt)
+            ix2 = sia.DeviceIndex(dt).build(s2, e2)
+            oix2 = orc.build_index(s2, e2)
+            want = orc.count_batch(oix2, qs2, qe2)
+            for ub_tree in (1, 0):
+                autotune.apply({"count_rank": 1, "offsets_rank": 1, "ub_tree": ub_tree})
+                assert np.array_equal(ix2.count(qs2, qe2), want), (dt, ub_tree)
+                off, idx = ix2.search(qs2, qe2)
+                ooff, oidx = orc.search_batch(oix2, qs2, qe2, mode=0)
+                assert np.array_equal(off, ooff) and np.array_equal(idx.astype(np.int64), oidx), (dt, ub_tree)
+            ix2.close()
+    finally:
+        autotune.apply(saved)
     # query before build
     with pytest.raises(sia.SIError):
         sia.DeviceIndex(np.int32).count(q, q)
