@@ -524,7 +524,7 @@ def main():
     if not a.no_tune:
         from superintervals_amd import autotune
         tuning = autotune.load_or_tune(ix, plan, nq, off, idx, total, stream=stream, retune=True,
-                                       verbose=a.verbose)
+                                       qs_t=tqs, qe_t=tqe, verbose=a.verbose)
 
     def step(evs=None):
         if evs:

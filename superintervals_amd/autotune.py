@@ -5,7 +5,8 @@ CU for the count and the fill pass, the per-lane step budget before a long walk 
 parked for the one-wave-per-query kernel, walk or rank form of the count); only their
 speed does.  tune() times every candidate on the caller's own index and planned batch
 with HIP events: first tile x workgroups for count + fill, then, with those fixed, the
-lane step budget by the fill's time and the count form by the count's.  Results are
+lane step budget by the fill's time, the count form by the count's and, given the
+queries, the plan's constant-digit check by the plan's.  Results are
 cached per workload signature (coordinate width, log2 of the index size, log2 of the
 batch size, log2 of the mean hits per query) in a JSON file, so later runs of the same
 shape reuse them.
@@ -80,12 +81,15 @@ def _time_ms(fn, reps, stream):
 
 
 def tune(ix, plan_t, nq, offsets_t, idx_t, stream=None, reps=3, tiles=CANDIDATE_TILES,
-         wgs=CANDIDATE_WGS, lane_steps=CANDIDATE_LANE_STEPS, verbose=False):
+         wgs=CANDIDATE_WGS, lane_steps=CANDIDATE_LANE_STEPS, qs_t=None, qe_t=None, verbose=False):
     """Time count + fill for every candidate on a planned batch; apply and return the best.
 
     ix is a DeviceIndex, plan_t a planned batch (plan_device), offsets_t the batch's
     offsets (offsets_planned already run) and idx_t an output buffer of offsets[nq]
-    entries.  Returns (best params, table of measured ms)."""
+    entries.  With the batch's query tensors (qs_t, qe_t) the plan's constant-digit check
+    (plan_sync: a digit histogram and one host read-back, which pay off when key digits
+    are constant, as in int64 coordinates) is timed both ways too; the plan is then redone
+    into plan_t.  Returns (best params, table of measured ms)."""
     import torch
     stream = stream or torch.cuda.current_stream()
     counts = torch.empty(nq, dtype=torch.int32, device=offsets_t.device)
@@ -135,6 +139,16 @@ def tune(ix, plan_t, nq, offsets_t, idx_t, stream=None, reps=3, tiles=CANDIDATE_
                 print("tune count_rank=%d: count %.3f ms" % (cr, tc), flush=True)
             if "count_rank" not in best or tc < best["count_ms"]:
                 best["count_rank"], best["count_ms"] = cr, tc
+        if qs_t is not None and qe_t is not None:
+            plan = lambda: ix.plan_device(qs_t, qe_t, plan_t, stream=stream)
+            for ps in (1, 0):
+                apply({"plan_sync": ps})
+                tp = _time_ms(plan, reps, stream)
+                results.append(dict(knobs_of(best), plan_sync=ps, plan_ms=tp))
+                if verbose:
+                    print("tune plan_sync=%d: plan %.3f ms" % (ps, tp), flush=True)
+                if "plan_sync" not in best or tp < best["plan_ms"]:
+                    best["plan_sync"], best["plan_ms"] = ps, tp
     finally:
         apply(saved)
     apply(knobs_of(best))
@@ -142,7 +156,7 @@ def tune(ix, plan_t, nq, offsets_t, idx_t, stream=None, reps=3, tiles=CANDIDATE_
 
 
 def load_or_tune(ix, plan_t, nq, offsets_t, idx_t, hits, cache=DEFAULT_CACHE, stream=None, retune=False,
-                 verbose=False):
+                 qs_t=None, qe_t=None, verbose=False):
     """Apply cached choices for this workload signature, tuning (and caching) if absent."""
     sig = signature(ix.dtype.itemsize, ix.n, nq, hits)
     table = _load(cache)
@@ -150,7 +164,7 @@ def load_or_tune(ix, plan_t, nq, offsets_t, idx_t, hits, cache=DEFAULT_CACHE, st
         apply(knobs_of(table[sig]))
         return dict(table[sig], cached=True, signature=sig)
     t0 = time.perf_counter()
-    best, _ = tune(ix, plan_t, nq, offsets_t, idx_t, stream=stream, verbose=verbose)
+    best, _ = tune(ix, plan_t, nq, offsets_t, idx_t, stream=stream, qs_t=qs_t, qe_t=qe_t, verbose=verbose)
     best = dict(best, tuned_s=round(time.perf_counter() - t0, 2))
     table[sig] = best
     try:

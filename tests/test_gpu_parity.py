@@ -338,12 +338,19 @@ def test_autotune_picks_and_keeps_results(sia, orc):
         total = int(off[-1].item())
         idx = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
         best, table = autotune.tune(ix, plan, nq, off, idx, reps=1, tiles=(512, 2048), wgs=(0,),
-                                    lane_steps=(16, 256))
+                                    lane_steps=(16, 256), qs_t=tqs, qe_t=tqe)
         assert best["lane_steps"] in (16, 256) and best["count_rank"] in (0, 1)
+        assert best["plan_sync"] in (0, 1)
         now = autotune.current()
         for k, v in autotune.knobs_of(best).items():
             assert now[k] == v, k
-        assert len(table) == 2 + 2 + 2
+        assert len(table) == 2 + 2 + 2 + 2
+        # the plan redone during tuning is still this batch's plan
+        ix.offsets_planned(plan, nq, off)
+        ix.fill_planned(plan, nq, off, idx, order="desc")
+        ooff, oidx = orc.search_batch(oix, qs, qe, mode=0)
+        assert np.array_equal(off.cpu().numpy().astype(np.uint64), ooff)
+        assert np.array_equal(idx[:total].cpu().numpy().view(np.uint32).astype(np.int64), oidx)
         counts = ix.count(qs, qe)
         assert np.array_equal(counts, orc.count_batch(oix, qs, qe))
         o, i = ix.search(qs, qe, order="desc")
