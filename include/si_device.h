@@ -120,12 +120,16 @@ double siIndexLastBuildMs(const siIndex* ix);
  *   "offsets_rank" 1 (default): siSearchOffsets* count with the rank form; 0: the walk
  *   "lane_steps"   line steps a lane spends on one query before the walk is parked for
  *                  the wave-cooperative kernel (1 .. 2^30, default 256)
- *   "ub_tree"      1 (default): siUpperBound descends a 64-ary tree of first keys (top
- *                  level of at most 4096 keys in Eytzinger order in LDS, then one 64-key
- *                  node per level down to the starts); 0: plain halving over all starts
+ *   "ub_tree"      1 (default): siUpperBound, the plan's upper_bound and the rank-form
+ *                  count descend 64-ary trees of first keys (top level of at most 4096
+ *                  keys in Eytzinger order in LDS, then one 64-key node per level) over
+ *                  the starts / the sorted ends; 0: plain halving
  *   "plan_sync"    1 (default): siPlanQueries reads the key-digit histograms back once to
  *                  skip constant digits (one stream synchronisation); 0: it sorts every
  *                  digit without a read-back, so the planned calls can be graph-captured
+ *   "plan_skip_low" 0 (default) .. 2: lowest 8-bit digits of the query end that
+ *                  siPlanQueries leaves unsorted (one radix pass fewer each; the walk
+ *                  allows the coarser order at some cost in LDS window hits)
  * superintervals_amd/autotune.py measures candidates and stores the best per workload. */
 int siTuneSet(const char* name, int64_t value);
 int64_t siTuneGet(const char* name);   /* -1 for an unknown name */

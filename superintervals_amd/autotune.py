@@ -6,7 +6,8 @@ parked for the one-wave-per-query kernel, walk or rank form of the count); only 
 speed does.  tune() times every candidate on the caller's own index and planned batch
 with HIP events: first tile x workgroups for count + fill, then, with those fixed, the
 lane step budget by the fill's time, the count form by the count's and, given the
-queries, the plan's constant-digit check by the plan's.  Results are
+queries, the plan's constant-digit check by the plan's and the unsorted low end digit
+by plan + fill.  Results are
 cached per workload signature (coordinate width, log2 of the index size, log2 of the
 batch size, log2 of the mean hits per query) in a JSON file, so later runs of the same
 shape reuse them.
@@ -21,7 +22,8 @@ import time
 from . import _native as N
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "superintervals_amd", "tune.json")
-KNOBS = ("tile", "count_wgs", "fill_wgs", "count_rank", "offsets_rank", "lane_steps", "plan_sync", "ub_tree")
+KNOBS = ("tile", "count_wgs", "fill_wgs", "count_rank", "offsets_rank", "lane_steps", "plan_sync", "ub_tree",
+         "plan_skip_low")
 CANDIDATE_TILES = (512, 1024, 2048, 4096)
 CANDIDATE_WGS = (1, 0)     # 0 = the occupancy limit
 CANDIDATE_LANE_STEPS = (64, 256, 1024)
@@ -149,6 +151,22 @@ def tune(ix, plan_t, nq, offsets_t, idx_t, stream=None, reps=3, tiles=CANDIDATE_
                     print("tune plan_sync=%d: plan %.3f ms" % (ps, tp), flush=True)
                 if "plan_sync" not in best or tp < best["plan_ms"]:
                     best["plan_sync"], best["plan_ms"] = ps, tp
+            # the lowest end digit left unsorted: one radix pass fewer, but coarser tiles;
+            # judged by plan + fill (the offsets do not depend on the plan's order)
+            apply({"plan_sync": best["plan_sync"]})
+            base = None
+            for psl in (0, 1):
+                apply({"plan_skip_low": psl})
+                tp = _time_ms(plan, reps, stream)
+                tf = _time_ms(fill, reps, stream)
+                results.append(dict(knobs_of(best), plan_skip_low=psl, plan_ms=tp, fill_ms=tf))
+                if verbose:
+                    print("tune plan_skip_low=%d: plan %.3f ms fill %.3f ms" % (psl, tp, tf), flush=True)
+                if base is None or tp + tf < base:
+                    base = tp + tf
+                    best["plan_skip_low"], best["plan_ms"], best["fill_ms"] = psl, tp, tf
+            apply({"plan_skip_low": best["plan_skip_low"]})
+            plan()   # plan_t holds the batch planned with the chosen knobs
     finally:
         apply(saved)
     apply(knobs_of(best))

@@ -7,11 +7,12 @@
 //
 // A batch runs in three stages.
 //  1. Plan: the queries are sorted by end (stable LSD radix sort, payload = query
-//     number) so the lanes of a wave walk neighbouring parts of the index and share
-//     cache lines; each sorted query gets its upper_bound position.
+//     number; optionally only down to a coarser digit) so the lanes of a wave walk
+//     neighbouring parts of the index and share cache lines; each sorted query gets
+//     its upper_bound position.
 //  2. Count: a persistent kernel walks every query.  A workgroup takes a tile of
 //     TILE consecutive sorted queries from a global counter, stages the window of
-//     ends[] and exitp[] just below the tile's largest upper_bound in LDS (64 KB),
+//     ends[] and exitp[] just below the tile's largest upper_bound in LDS (26-27 KB),
 //     and walks from LDS; the few reads below the window go to HBM.  One lane owns
 //     one query; when it finishes, the lane takes the next query from a pool of 64
 //     the wave grabs from the tile's LDS counter, so lanes stay busy however uneven
@@ -212,6 +213,10 @@ __global__ __launch_bounds__(Q_THREADS) void k_count_rank(IndexView<S> ix, const
 //   plan_sync    1: the plan's radix sort reads the digit histograms back to skip constant
 //                digits (one stream synchronisation); 0: it sorts every key digit without
 //                a host read-back, so plan -> offsets -> fill can be captured in a graph
+//   plan_skip_low  lowest 8-bit digits of the query end the plan leaves unsorted (0..2):
+//                one radix pass fewer each; queries are then in end order only up to
+//                those bits, which the walk's tiles allow (their LDS window ends at the
+//                tile's largest upper_bound) at some cost in window hits
 struct Tuning {
     uint32_t tile = 1024;
     int count_wgs = 0;
@@ -221,6 +226,7 @@ struct Tuning {
     uint32_t lane_steps = 256;
     int plan_sync = 1;
     int ub_tree = 1;
+    int plan_skip_low = 0;
 };
 Tuning g_tune;
 
@@ -240,11 +246,11 @@ int plan_impl(const siIndex* ix, const S* qs, const S* qe, size_t nq, void* plan
     if (sizeof(S) == 4) {
         k_plan_keys_i32<<<g, Q_THREADS, 0, st>>>((const int32_t*)qs, (const int32_t*)qe, nq, k0, v0);
         LAUNCH_CHECK();
-        rc = radix_sort_pairs(k0, v0, k1, v1, nq, 4, 8, rt, &ks, &vs, st, g_tune.plan_sync != 0);
+        rc = radix_sort_pairs(k0, v0, k1, v1, nq, 4 + g_tune.plan_skip_low, 8, rt, &ks, &vs, st, g_tune.plan_sync != 0);
     } else {
         k_plan_keys_i64<<<g, Q_THREADS, 0, st>>>((const int64_t*)qe, nq, k0, v0);
         LAUNCH_CHECK();
-        rc = radix_sort_pairs(k0, v0, k1, v1, nq, 0, 8, rt, &ks, &vs, st, g_tune.plan_sync != 0);
+        rc = radix_sort_pairs(k0, v0, k1, v1, nq, g_tune.plan_skip_low, 8, rt, &ks, &vs, st, g_tune.plan_sync != 0);
     }
     if (rc != SI_OK) return rc;
     // upper_bound of the sorted ends: through the search tree when the index has one
@@ -463,7 +469,7 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
     constexpr uint32_t WIN = TileCfg<S>::WIN;
     __shared__ __align__(16) S s_end[WIN];
     __shared__ __align__(16) uint32_t s_ex[WIN];   // exit targets (IndexView::exitp)
-    __shared__ uint32_t s_tile, s_next;
+    __shared__ uint32_t s_tile, s_next, s_whi1;
     // per-wave emission slots: 64 + one for the segment carried between chunks
     __shared__ EmitSlot s_emit[(MODE == W_COUNT) ? 1 : T_THREADS / WAVE * (WAVE + 1)];
     __shared__ uint32_t s_kbit[(MODE == W_COUNT || !SI_EMIT_LUT) ? 1 : 256];   // byte set-bit table
@@ -482,13 +488,24 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
 
     while (true) {
         // ---- take the next tile (workgroup-level work stealing)
-        if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1u);
+        if (threadIdx.x == 0) {
+            s_tile = atomicAdd(tile_counter, 1u);
+            s_whi1 = 0u;
+        }
         __syncthreads();
         const uint32_t t = s_tile;
         if (t >= ntiles) break;
         const uint32_t t0 = t * tile, t1 = min(t0 + tile, nq);
-        // sorted by end, so the last query of the tile has the largest upper_bound
-        const uint32_t whi = pv.idx[t1 - 1];
+        // the window ends at the tile's largest upper_bound (the plan's order by end need
+        // not be exact): max of idx + 1 over the tile, where NONE + 1 = 0 drops queries
+        // that have no start <= their end
+        {
+            uint32_t mx = 0;
+            for (uint32_t k = t0 + threadIdx.x; k < t1; k += T_THREADS) mx = max(mx, pv.idx[k] + 1u);
+            if (mx) atomicMax(&s_whi1, mx);
+        }
+        __syncthreads();
+        const uint32_t whi = s_whi1 - 1u;
         uint32_t wlo = 0, wcnt = 0;
         if (whi != NONE) {
             const int64_t lo = (int64_t)(whi & ~15u) + 16 - (int64_t)WIN;
@@ -1129,6 +1146,9 @@ SI_EXPORT int siTuneSet(const char* name, int64_t value) {
     } else if (k == "plan_sync") {
         if (value != 0 && value != 1) return si_set_error(SI_ERR_ARG, "plan_sync must be 0 or 1");
         g_tune.plan_sync = (int)value;
+    } else if (k == "plan_skip_low") {
+        if (value < 0 || value > 2) return si_set_error(SI_ERR_ARG, "plan_skip_low must be in [0, 2]");
+        g_tune.plan_skip_low = (int)value;
     } else if (k == "count_rank" || k == "offsets_rank") {
         if (value != 0 && value != 1) return si_set_error(SI_ERR_ARG, "%s must be 0 or 1", name);
         (k == "count_rank" ? g_tune.count_rank : g_tune.offsets_rank) = (int)value;
@@ -1147,6 +1167,7 @@ SI_EXPORT int64_t siTuneGet(const char* name) {
     if (k == "lane_steps") return g_tune.lane_steps;
     if (k == "plan_sync") return g_tune.plan_sync;
     if (k == "ub_tree") return g_tune.ub_tree;
+    if (k == "plan_skip_low") return g_tune.plan_skip_low;
     if (k == "count_rank") return g_tune.count_rank;
     if (k == "offsets_rank") return g_tune.offsets_rank;
     return -1;

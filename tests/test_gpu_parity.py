@@ -241,11 +241,12 @@ def test_launch_choices_do_not_change_results(sia, orc):
     ix = _dev_index(sia, s, e)
     saved = autotune.current()
     try:
-        for tile, cw, fw, cr, orank in ((64, 1, 1, 0, 1), (512, 0, 1, 1, 0), (4096, 1, 0, 1, 1),
-                                        (100000, 0, 0, 0, 0)):
-            print(f"[gpu] launch tile={tile} count_wgs={cw} fill_wgs={fw} rank={cr}/{orank}", flush=True)
+        for tile, cw, fw, cr, orank, psl in ((64, 1, 1, 0, 1, 0), (512, 0, 1, 1, 0, 1), (4096, 1, 0, 1, 1, 2),
+                                             (100000, 0, 0, 0, 0, 1), (1024, 0, 0, 1, 1, 2)):
+            print(f"[gpu] launch tile={tile} count_wgs={cw} fill_wgs={fw} rank={cr}/{orank} skip_low={psl}",
+                  flush=True)
             autotune.apply({"tile": tile, "count_wgs": cw, "fill_wgs": fw, "count_rank": cr,
-                            "offsets_rank": orank})
+                            "offsets_rank": orank, "plan_skip_low": psl})
             assert np.array_equal(ix.count(qs, qe), rec["counts"])
             off, idx = ix.search(qs, qe, order="vector")
             assert orc.digest(orc.hash_csr(off, idx)) == int(rec["hash_vec_digest"])
@@ -340,11 +341,11 @@ def test_autotune_picks_and_keeps_results(sia, orc):
         best, table = autotune.tune(ix, plan, nq, off, idx, reps=1, tiles=(512, 2048), wgs=(0,),
                                     lane_steps=(16, 256), qs_t=tqs, qe_t=tqe)
         assert best["lane_steps"] in (16, 256) and best["count_rank"] in (0, 1)
-        assert best["plan_sync"] in (0, 1)
+        assert best["plan_sync"] in (0, 1) and best["plan_skip_low"] in (0, 1)
         now = autotune.current()
         for k, v in autotune.knobs_of(best).items():
             assert now[k] == v, k
-        assert len(table) == 2 + 2 + 2 + 2
+        assert len(table) == 2 + 2 + 2 + 2 + 2
         # the plan redone during tuning is still this batch's plan
         ix.offsets_planned(plan, nq, off)
         ix.fill_planned(plan, nq, off, idx, order="desc")
