@@ -204,8 +204,13 @@ int device_excl_scan(const TI* in, size_t n, TO* out, TO* total, TO* ws, hipStre
 // Radix sort: stable LSD over 64-bit keys with a 32-bit payload
 // ============================================================================
 constexpr int RS_THREADS = 256;
-constexpr int RS_ROUNDS = 16;
-constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;   // 4096 items per tile
+// 2048-item tiles: the scatter's LDS (keys, payloads, digit counts: 30 KB) lets five
+// blocks share a CU; 4096-item tiles (54 KB, two blocks) were 10 % slower per pass
+#ifndef SI_RS_ROUNDS
+#define SI_RS_ROUNDS 8
+#endif
+constexpr int RS_ROUNDS = SI_RS_ROUNDS;   // rounds of 256 items per tile
+constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;   // items per tile
 constexpr int RS_BINS = 256;
 
 // Histograms of digits [dlo, dhi) at once (to skip constant-digit passes).
