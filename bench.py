@@ -294,7 +294,9 @@ def bench_skewed(a):
         return e0.elapsed_time(e1) / reps
 
     try:
-        autotune.apply({"count_rank": 0})
+        autotune.apply({"count_rank": 0, "heavy_ends32": 0})
+        t_walk64 = timed(lambda: ix.count_planned(plan, nq, counts, stream=stream))
+        autotune.apply({"heavy_ends32": saved["heavy_ends32"]})
         t_walk = timed(lambda: ix.count_planned(plan, nq, counts, stream=stream))
         c_walk = counts.clone()
         autotune.apply({"count_rank": 1})
@@ -304,8 +306,8 @@ def bench_skewed(a):
         autotune.apply(saved)
     hits = int(c_walk.to(torch.int64).sum().item())
     wide = int((c_walk > 100_000).sum().item())
-    print("[skewed] count: walk %.2f ms, rank %.2f ms, identical %s, %.3g hits" % (t_walk, t_rank, same, hits),
-          file=sys.stderr, flush=True)
+    print("[skewed] count: walk %.2f ms (int64 ends in the wave walk: %.2f ms), rank %.2f ms, identical %s, %.3g hits"
+          % (t_walk, t_walk64, t_rank, same, hits), file=sys.stderr, flush=True)
     # streamed search: batches of bq queries through plan -> offsets -> fill (descending)
     nb = (nq + bq - 1) // bq
     bplan = torch.empty(ix.plan_bytes(bq), dtype=torch.uint8, device=dev)
@@ -332,6 +334,18 @@ def bench_skewed(a):
         autotune.apply(saved)
     best_ls = min(sweep, key=sweep.get)
     autotune.apply({"lane_steps": best_ls})
+    # the wave walk's ends: the 32-bit copy or the int64 array, by the fill of one batch
+    ends32_ms = {}
+    try:
+        for e32 in (1, 0):
+            autotune.apply({"heavy_ends32": e32})
+            ends32_ms[e32] = timed(lambda: ix.fill_planned(bplan, m0, off, idx, order="desc", stream=stream), reps=1)
+    finally:
+        autotune.apply({"heavy_ends32": saved["heavy_ends32"]})
+    best_e32 = min(ends32_ms, key=ends32_ms.get)
+    autotune.apply({"heavy_ends32": best_e32})
+    print("[skewed] fill of one batch: 32-bit ends in the wave walk %.1f ms, int64 ends %.1f ms"
+          % (ends32_ms[1], ends32_ms[0]), file=sys.stderr, flush=True)
     t_fill = 0.0
     tot = 0
     t0 = time.perf_counter()
@@ -365,10 +379,12 @@ def bench_skewed(a):
            "config": {"workload": "%dM int64 intervals (hotspot mixture, 20 %% long) x %dM queries (1 %% wide)" % (n // 10**6, nq // 10**6),
                       "intervals": n, "queries": nq, "search_batch": bq, "host_generation_s": round(gen_s, 2),
                       "build_ms": round(ix.last_build_ms, 2)},
-           "plan_ms": t_plan, "count_walk_ms": t_walk, "count_rank_ms": t_rank, "count_identical": same,
+           "plan_ms": t_plan, "count_walk_ms": t_walk, "count_walk_int64_ends_ms": t_walk64,
+           "count_rank_ms": t_rank, "count_identical": same,
            "hits": hits, "queries_over_1e5_hits": wide,
            "lane_steps_sweep_ms": {str(k): v for k, v in sweep.items()}, "lane_steps": best_ls,
            "lane_steps_count_walk_ms": {str(k): v for k, v in csweep.items()},
+           "fill_batch_ms_ends32": ends32_ms.get(1), "fill_batch_ms_int64_ends": ends32_ms.get(0),
            "search": {"batches": nb, "fill_ms_total": t_fill, "positions_per_s": tot / (t_fill / 1e3),
                       "wall_s_including_plan_offsets_sync": wall}}
     print(json.dumps(out))

@@ -127,6 +127,9 @@ double siIndexLastBuildMs(const siIndex* ix);
  *   "plan_sync"    1 (default): siPlanQueries reads the key-digit histograms back once to
  *                  skip constant digits (one stream synchronisation); 0: it sorts every
  *                  digit without a read-back, so the planned calls can be graph-captured
+ *   "heavy_ends32" 1 (default): the wave-cooperative walk of an int64 index whose ends
+ *                  span less than 2^32 reads a 32-bit copy of them (8 B per position with
+ *                  exitp instead of 12); 0: the int64 ends
  *   "plan_skip_low" 0 (default) .. 2: lowest 8-bit digits of the query end that
  *                  siPlanQueries leaves unsorted (one radix pass fewer each; the walk
  *                  allows the coarser order at some cost in LDS window hits)

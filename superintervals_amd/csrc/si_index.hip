@@ -227,6 +227,13 @@ __global__ __launch_bounds__(256) void k_branch(TreeDesc<S> td, uint32_t* __rest
 
 namespace {
 
+// ends[i] - base as uint32 (the caller checked that every end fits), 0 in the padding.
+__global__ __launch_bounds__(256) void k_ends32(const int64_t* __restrict__ ends, size_t n, size_t npad, int64_t base,
+                                                uint32_t* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < npad) out[i] = (i < n) ? (uint32_t)((uint64_t)ends[i] - (uint64_t)base) : 0u;
+}
+
 // Search tree over n sorted keys: 64-ary levels of first keys up to a top level of at
 // most 4095 keys, which is also stored in Eytzinger order (k_every64, k_eytz).
 template <typename S>
@@ -417,6 +424,22 @@ int build_impl(siIndex* ix, const void* starts, const void* ends, const int32_t*
     {
         const int rc = build_key_tree<S>((const S*)ix->d_ends_sorted, n, ix->eb, st);
         if (rc != SI_OK) return rc;
+    }
+    // 32-bit ends for the wave-cooperative walk of int64 indexes whose ends span < 2^32:
+    // a third fewer bytes per position than ends + exitp at 8 + 4
+    if (sizeof(S) == 8) {
+        int64_t lohi[2];
+        HIP_TRY(hipMemcpyAsync(&lohi[0], (const int64_t*)ix->d_ends_sorted, 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(&lohi[1], (const int64_t*)ix->d_ends_sorted + (n - 1), 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if ((uint64_t)lohi[1] - (uint64_t)lohi[0] < 0xFFFFFFFFull) {
+            const size_t npad = (n + 63) & ~(size_t)63;
+            HIP_TRY(hipMalloc(&ix->d_ends32, npad * sizeof(uint32_t)));
+            ix->ends32_base = lohi[0];
+            k_ends32<<<grid_for(npad, 256), 256, 0, st>>>((const int64_t*)ix->d_ends, n, npad, lohi[0],
+                                                           ix->d_ends32);
+            LAUNCH_CHECK();
+        }
     }
     HIP_TRY(hipStreamSynchronize(st));
     if (!on_device) { hipFree(in_s); hipFree(in_e); if (in_v) hipFree(in_v); }

@@ -43,6 +43,8 @@ struct siIndex {
     int32_t* d_values = nullptr;
     void* d_tree = nullptr;   // levels 1.. of the max tree, contiguous
     void* d_ends_sorted = nullptr;   // ends in ascending order (rank count)
+    uint32_t* d_ends32 = nullptr;    // int64 only, ends - ends32_base when they span < 2^32
+    int64_t ends32_base = 0;
     KeyTree ub;   // search tree over the starts (upper_bound)
     KeyTree eb;   // search tree over the sorted ends (rank count)
     int nlev = 0;
@@ -52,8 +54,10 @@ struct siIndex {
 
     void release() {
         hipFree(d_starts); hipFree(d_ends); hipFree(d_branch); hipFree(d_exit); hipFree(d_perm);
-        hipFree(d_values); hipFree(d_tree); hipFree(d_ends_sorted);
+        hipFree(d_values); hipFree(d_tree); hipFree(d_ends_sorted); hipFree(d_ends32);
         d_starts = d_ends = d_tree = d_ends_sorted = nullptr;
+        d_ends32 = nullptr;
+        ends32_base = 0;
         ub.release();
         eb.release();
         d_branch = d_exit = d_perm = nullptr;
@@ -70,6 +74,8 @@ struct siIndex {
         v.ends = (const S*)d_ends;
         v.branch = d_branch;
         v.exitp = d_exit;
+        v.ends32 = d_ends32;
+        v.ends32_base = ends32_base;
         v.n = (uint32_t)n;
         v.npad = (uint32_t)((n + 63) & ~(size_t)63);
         return v;

@@ -11,6 +11,8 @@
 //   values[n]            int32 payload in sorted order (optional)
 //   tree levels          S, a 64-ary max tree over ends: level l+1 holds the max of
 //                        each group of 64 entries of level l (level 0 is ends)
+//   ends32[n]            uint32, int64 indexes whose ends span less than 2^32 only:
+//                        ends[i] - ends32_base, read by the wave-cooperative walk (padded)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -36,6 +38,8 @@ struct IndexView {
     const S* __restrict__ ends;
     const uint32_t* __restrict__ branch;
     const uint32_t* __restrict__ exitp;   // line exit target of each position (see k_exit)
+    const uint32_t* __restrict__ ends32;  // ends - ends32_base, or null
+    int64_t ends32_base;
     uint32_t n;
     uint32_t npad;   // allocated length of starts/ends/branch (multiple of 64)
 };

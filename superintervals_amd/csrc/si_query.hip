@@ -213,6 +213,8 @@ __global__ __launch_bounds__(Q_THREADS) void k_count_rank(IndexView<S> ix, const
 //   plan_sync    1: the plan's radix sort reads the digit histograms back to skip constant
 //                digits (one stream synchronisation); 0: it sorts every key digit without
 //                a host read-back, so plan -> offsets -> fill can be captured in a graph
+//   heavy_ends32 1: the wave-cooperative walk of an int64 index reads the 32-bit copy of
+//                the ends when the build made one (ends spanning < 2^32); 0: the ends
 //   plan_skip_low  lowest 8-bit digits of the query end the plan leaves unsorted (0..2):
 //                one radix pass fewer each; queries are then in end order only up to
 //                those bits, which the walk's tiles allow (their LDS window ends at the
@@ -227,6 +229,7 @@ struct Tuning {
     int plan_sync = 1;
     int ub_tree = 1;
     int plan_skip_low = 0;
+    int heavy_ends32 = 1;
 };
 Tuning g_tune;
 
@@ -728,7 +731,10 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanV
             for (uint32_t l = 0; l < HEAVY_LINES; ++l) {
                 // lines below position 0 are never stepped into; line 0 stands in for them
                 const uint32_t lb = (tb >= 64u * l) ? tb - 64u * l : 0u;
-                cv[l] = ix.ends[lb + lane];
+                if (sizeof(S) == 8 && ix.ends32)   // int64 ends within 2^32 of their minimum
+                    cv[l] = (S)((int64_t)ix.ends32[lb + lane] + ix.ends32_base);
+                else
+                    cv[l] = ix.ends[lb + lane];
                 cx[l] = ix.exitp[lb + lane];
             }
             // the walk leaves every line it steps into and only moves down, so each fetched
@@ -1093,7 +1099,9 @@ int launch_walk(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         cus = std::max(cus, 1);
     }
-    k_walk_heavy<S, MODE><<<(unsigned)cus * 8u, Q_THREADS, 0, st>>>(ix->view<S>(), pv, counts, counts64, offsets,
+    IndexView<S> hv = ix->view<S>();
+    if (!g_tune.heavy_ends32) hv.ends32 = nullptr;
+    k_walk_heavy<S, MODE><<<(unsigned)cus * 8u, Q_THREADS, 0, st>>>(hv, pv, counts, counts64, offsets,
                                                                    out_idx, ix->d_values, out_val, cap);
     LAUNCH_CHECK();
     return SI_OK;
@@ -1146,6 +1154,9 @@ SI_EXPORT int siTuneSet(const char* name, int64_t value) {
     } else if (k == "plan_sync") {
         if (value != 0 && value != 1) return si_set_error(SI_ERR_ARG, "plan_sync must be 0 or 1");
         g_tune.plan_sync = (int)value;
+    } else if (k == "heavy_ends32") {
+        if (value != 0 && value != 1) return si_set_error(SI_ERR_ARG, "heavy_ends32 must be 0 or 1");
+        g_tune.heavy_ends32 = (int)value;
     } else if (k == "plan_skip_low") {
         if (value < 0 || value > 2) return si_set_error(SI_ERR_ARG, "plan_skip_low must be in [0, 2]");
         g_tune.plan_skip_low = (int)value;
@@ -1168,6 +1179,7 @@ SI_EXPORT int64_t siTuneGet(const char* name) {
     if (k == "plan_sync") return g_tune.plan_sync;
     if (k == "ub_tree") return g_tune.ub_tree;
     if (k == "plan_skip_low") return g_tune.plan_skip_low;
+    if (k == "heavy_ends32") return g_tune.heavy_ends32;
     if (k == "count_rank") return g_tune.count_rank;
     if (k == "offsets_rank") return g_tune.offsets_rank;
     return -1;

@@ -281,15 +281,20 @@ def test_heavy_path_parity(sia, orc, lane_steps):
     ix.close()
 
 
-@pytest.mark.parametrize("lane_steps", [1, 40])
-def test_heavy_path_int64_long_walks(sia, orc, lane_steps):
+@pytest.mark.parametrize("lane_steps,far", [(1, False), (40, False), (1, True)])
+def test_heavy_path_int64_long_walks(sia, orc, lane_steps, far):
     """int64 hotspot mixture (workload 3 distributions): every walk parked after lane_steps
     lines and finished by the wave-cooperative kernel, which fetches four lines at a time
-    and leaves a line through exitp lookups.  Counts, both orders and values equal the
-    oracle's."""
+    and leaves a line through exitp lookups.  Ends within 2^32 of each other are read from
+    the 32-bit copy; far = True adds intervals 2^40 away, so the index has no such copy.
+    Counts, both orders and values equal the oracle's."""
     from superintervals_amd import autotune
-    print(f"[gpu] heavy path int64 lane_steps={lane_steps}", flush=True)
+    print(f"[gpu] heavy path int64 lane_steps={lane_steps} far={far}", flush=True)
     s, e = sia.workload_intervals(3, 1_000_000, seed=33, sorted=False)
+    if far:
+        extra = (np.int64(1) << 40) + np.arange(0, 2000 * 97, 97, dtype=np.int64)
+        s = np.concatenate([s, extra])
+        e = np.concatenate([e, extra + 500])
     qs, qe = sia.workload_queries(3, 6_000, seed=34)
     # ragged edges: queries before, after and around the whole index, and a point query
     qs = np.concatenate([qs, np.array([-5, 0, int(e.max()), int(s.min()), 10**9], np.int64)])
