@@ -231,6 +231,32 @@ int siCoverage(const siIndex* ix, const void* qs, const void* qe, size_t nq, uin
 int siGatherKeys(const siIndex* ix, const uint32_t* idx, size_t n, void* out_starts, void* out_ends,
                  int32_t* out_values, void* stream);
 
+/* ---- Set operations over a batched search ---------------------------------------
+ * A's intervals k (as, ae, optional data ad, in the caller's order, which is the order of
+ * the output) are queries against the index ix (the second set B).
+ *
+ * siIntersectPieces replaces superintervals.hpp:1158-1174 (intersection): with offsets /
+ * idx from a search of A's intervals in ix in the C++ vector order (siSearchFill with
+ * SI_ORDER_VECTOR), output j (j in [offsets[k], offsets[k+1])) is the overlap
+ * [max(as[k], starts[h]), min(ae[k], ends[h])] of A's interval k with B's interval h =
+ * idx[j]; out_adata[j] = ad[k] (the default combine keeps A's data), out_bdata[j] =
+ * values[h].  Any output may be NULL.
+ *
+ * siDifferenceOffsets / siDifferenceFill replace superintervals.hpp:1188-1211
+ * (difference): with offsets / idx from a descending search (SI_ORDER_DESC), the first
+ * writes the CSR piece_offsets[na + 1] of the uncovered pieces of each A interval
+ * (workspace of siSearchWorkspaceBytes(na) bytes), the second writes the pieces
+ * [out_starts, out_ends] with A's data, in the reference's order. */
+int siIntersectPieces(const siIndex* ix, const void* as, const void* ae, const int32_t* ad, size_t na,
+                      const uint64_t* offsets, const uint32_t* idx, void* out_starts, void* out_ends,
+                      int32_t* out_adata, int32_t* out_bdata, void* stream);
+int siDifferenceOffsets(const siIndex* ix, const void* as, const void* ae, size_t na, const uint64_t* offsets,
+                        const uint32_t* idx, uint64_t* piece_offsets, void* workspace, size_t workspace_bytes,
+                        void* stream);
+int siDifferenceFill(const siIndex* ix, const void* as, const void* ae, const int32_t* ad, size_t na,
+                     const uint64_t* offsets, const uint32_t* idx, const uint64_t* piece_offsets,
+                     void* out_starts, void* out_ends, int32_t* out_data, void* stream);
+
 /* Reference-walk statistics summed over the batch: totals[0] = hits, totals[1] = reads of
  * ends[] by the walk, totals[2] = reads of branch[] (jumps), totals[3] = queries whose
  * upper_bound found a position.  The walk is c_superintervals.h:610-643. */

@@ -245,6 +245,59 @@ class DeviceIndex:
             out.append(kv[:total].cpu().numpy())
         return tuple(out)
 
+    def _set_op_inputs(self, a_s, a_e, order):
+        import torch
+        a_s, a_e = self._q(a_s, a_e)
+        off, idx = self.search(a_s, a_e, order=order)
+        dev = torch.device("cuda:0")
+        t = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+        return a_s, a_e, off, idx, t, dev
+
+    def intersection_pieces(self, a_s, a_e, a_data=None):
+        """Intersection of the intervals (a_s[k], a_e[k]) with this index, superintervals.hpp
+        :1158-1174: for each k in order and each hit h in the C++ vector order, the piece
+        [max(a_s, starts[h]), min(a_e, ends[h])].  Returns (starts, ends, k, h): the source
+        interval k of the first set and the sorted position h in this index of each piece."""
+        import torch
+        a_s, a_e, off, idx, t, dev = self._set_op_inputs(a_s, a_e, "vector")
+        na, total = a_s.size, int(off[-1])
+        tdt = torch.int32 if self.dtype == np.int32 else torch.int64
+        os_, oe = (torch.empty(max(total, 1), dtype=tdt, device=dev) for _ in range(2))
+        ok = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+        if total:
+            ts, te, toff, tidx = t(a_s), t(a_e), t(off.view(np.int64)), t(idx)
+            tk = torch.arange(na, dtype=torch.int32, device=dev)
+            N.check(N.lib.siIntersectPieces(self._h, _ptr(ts), _ptr(te), _ptr(tk), na, _ptr(toff), _ptr(tidx),
+                                            _ptr(os_), _ptr(oe), _ptr(ok), None, None), "siIntersectPieces")
+            torch.cuda.synchronize()
+        return (os_[:total].cpu().numpy(), oe[:total].cpu().numpy(), ok[:total].cpu().numpy(),
+                idx[:total].astype(np.int64))
+
+    def difference_pieces(self, a_s, a_e):
+        """Difference of the intervals (a_s[k], a_e[k]) and this index, superintervals.hpp
+        :1188-1211: the parts of each interval k that no interval here covers, in the
+        reference's order.  Returns (starts, ends, k)."""
+        import torch
+        a_s, a_e, off, idx, t, dev = self._set_op_inputs(a_s, a_e, "desc")
+        na = a_s.size
+        tdt = torch.int32 if self.dtype == np.int32 else torch.int64
+        ts, te, toff = t(a_s), t(a_e), t(off.view(np.int64))
+        tidx = t(idx) if idx.size else torch.empty(1, dtype=torch.int32, device=dev)
+        poff = torch.empty(na + 1, dtype=torch.int64, device=dev)
+        ws = torch.empty(max(self.workspace_bytes(na), 256), dtype=torch.uint8, device=dev)
+        N.check(N.lib.siDifferenceOffsets(self._h, _ptr(ts), _ptr(te), na, _ptr(toff), _ptr(tidx), _ptr(poff),
+                                          _ptr(ws), ws.numel(), None), "siDifferenceOffsets")
+        torch.cuda.synchronize()
+        total = int(poff[-1].item())
+        os_, oe = (torch.empty(max(total, 1), dtype=tdt, device=dev) for _ in range(2))
+        ok = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+        if total:
+            tk = torch.arange(na, dtype=torch.int32, device=dev)
+            N.check(N.lib.siDifferenceFill(self._h, _ptr(ts), _ptr(te), _ptr(tk), na, _ptr(toff), _ptr(tidx),
+                                           _ptr(poff), _ptr(os_), _ptr(oe), _ptr(ok), None), "siDifferenceFill")
+            torch.cuda.synchronize()
+        return os_[:total].cpu().numpy(), oe[:total].cpu().numpy(), ok[:total].cpu().numpy()
+
     def walk_stats_device(self, qs_t, qe_t, stream=None):
         """(hits, ends reads, branch reads, found) totals of the reference walk."""
         import torch
@@ -376,7 +429,8 @@ class IntervalMap:
     search_idxs / search_idxs_batch return the C++ vector order (leading contiguous run
     ascending, then descending, superintervals.hpp:879-905) and search_values the
     strictly descending order.  The batch calls return numpy arrays (count_batch) and
-    lists of numpy arrays instead of Python lists.  Set operations are not provided.
+    lists of numpy arrays instead of Python lists.  intersection and difference follow
+    superintervals.hpp:1158-1211 (the C++ set operations; the first set need not be built).
     """
 
     def __init__(self):
@@ -557,3 +611,42 @@ class IntervalMap:
         o = off.tolist()
         il = idx.tolist()
         return [[vals[k] for k in il[o[q]:o[q + 1]]] for q in range(starts.shape[0])]
+
+    # ---- set operations (superintervals.hpp:1158-1211) ------------------------
+    def _set_arrays(self):
+        """This set's intervals in its current order (as added when not built, else sorted)."""
+        if self._dirty:
+            return (np.asarray(self._starts, np.int32), np.asarray(self._ends, np.int32), self._vals)
+        return self._s, self._e, self._vals_sorted
+
+    def intersection(self, other):
+        """Regions covered by both sets (superintervals.hpp:1158-1174): a new, unindexed
+        IntervalMap with every overlapping piece of this set's interval k and other's
+        interval h, for k in this set's order and h in search_idxs order, carrying k's
+        value.  other must be built."""
+        other._need_built()
+        s, e, vals = self._set_arrays()
+        out = IntervalMap()
+        if s.size == 0 or other.size() == 0:
+            return out
+        ps, pe, pk, _ = other._ix.intersection_pieces(s, e)
+        for a, b, k in zip(ps.tolist(), pe.tolist(), pk.tolist()):
+            out.add(a, b, vals[k])
+        return out
+
+    def difference(self, other):
+        """Regions of this set that other does not cover (superintervals.hpp:1188-1211): a
+        new, unindexed IntervalMap of the uncovered pieces of each interval, in order, with
+        its value.  other must be built."""
+        other._need_built()
+        s, e, vals = self._set_arrays()
+        out = IntervalMap()
+        if s.size == 0:
+            return out
+        if other.size() == 0:
+            ps, pe, pk = s, e, np.arange(s.size)
+        else:
+            ps, pe, pk = other._ix.difference_pieces(s, e)
+        for a, b, k in zip(ps.tolist(), pe.tolist(), pk.tolist()):
+            out.add(a, b, vals[k])
+        return out

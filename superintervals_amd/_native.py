@@ -65,6 +65,9 @@ _SIGS = {
     "siCoverage": (_i, [_vp, _vp, _vp, _sz, _vp, _vp, _vp]),
     "siWalkStats": (_i, [_vp, _vp, _vp, _sz, _vp, _vp]),
     "siGatherKeys": (_i, [_vp, _vp, _sz, _vp, _vp, _vp, _vp]),
+    "siIntersectPieces": (_i, [_vp, _vp, _vp, _vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "siDifferenceOffsets": (_i, [_vp, _vp, _vp, _sz, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "siDifferenceFill": (_i, [_vp, _vp, _vp, _vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "siLearnedCreate": (_vp, [_vp, C.c_uint32]),
     "siLearnedDestroy": (None, [_vp]),
     "siLearnedTrain": (_i, [_vp, _i, C.c_uint32, C.c_float, _vp]),

@@ -1390,6 +1390,146 @@ SI_EXPORT int siGatherKeys(const siIndex* ix, const uint32_t* idx, size_t n, voi
     }()));
 }
 
+// ---- Set operations over a batched search (superintervals.hpp:1158-1211) -------------
+// Query k is interval k of the first set A (its starts, ends, data as the caller passes
+// them); positions are the hits of a search of A's intervals in index B.
+
+// Intersection pieces (superintervals.hpp:1158-1174): hit j of query k (search_idxs
+// vector order) gives [max(a_s, b_s), min(a_e, b_e)] with A's data (the default combine)
+// and B's, one thread per query.
+template <typename S>
+__global__ __launch_bounds__(Q_THREADS) void k_intersect_pieces(IndexView<S> ix, const int32_t* __restrict__ bvals,
+                                                                const S* __restrict__ as, const S* __restrict__ ae,
+                                                                const int32_t* __restrict__ ad, size_t na,
+                                                                const uint64_t* __restrict__ off,
+                                                                const uint32_t* __restrict__ idx,
+                                                                S* __restrict__ os, S* __restrict__ oe,
+                                                                int32_t* __restrict__ oad, int32_t* __restrict__ obd) {
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= na) return;
+    const S s = as[k], e = ae[k];
+    const int32_t d = ad ? ad[k] : 0;
+    for (uint64_t j = off[k]; j < off[k + 1]; ++j) {
+        const uint32_t h = idx[j];
+        const S bs = ix.starts[h], be = ix.ends[h];
+        if (os) os[j] = s > bs ? s : bs;
+        if (oe) oe[j] = e < be ? e : be;
+        if (oad) oad[j] = d;
+        if (obd) obd[j] = bvals ? bvals[h] : 0;
+    }
+}
+
+// Difference pieces (superintervals.hpp:1188-1211): the uncovered parts of A's interval k.
+// The reference sorts B's hits ascending by (start, end); a descending search lists them
+// in exactly the reverse of that order (positions are sorted by start asc, end desc), so
+// the sweep reads the hits backwards.  Once a hit covers A's interval to its end nothing
+// more is uncovered (every later piece would start past the end), so the sweep stops
+// there instead of stepping the cursor to end + 1, which would wrap at the type's
+// maximum.  FILL = false counts the pieces, true writes them.
+template <typename S, bool FILL>
+__global__ __launch_bounds__(Q_THREADS) void k_difference_pieces(IndexView<S> ix, const S* __restrict__ as,
+                                                                 const S* __restrict__ ae,
+                                                                 const int32_t* __restrict__ ad, size_t na,
+                                                                 const uint64_t* __restrict__ off,
+                                                                 const uint32_t* __restrict__ idx,
+                                                                 uint64_t* __restrict__ pieces,
+                                                                 S* __restrict__ os, S* __restrict__ oe,
+                                                                 int32_t* __restrict__ od) {
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= na) return;
+    const int64_t s = (int64_t)as[k], e = (int64_t)ae[k];
+    const int32_t d = ad ? ad[k] : 0;
+    uint64_t w = FILL ? pieces[k] : 0;
+    int64_t cursor = s;
+    bool past = false;   // cursor is beyond e
+    for (uint64_t j = off[k + 1]; j-- > off[k];) {
+        const uint32_t h = idx[j];
+        const int64_t cs = max((int64_t)ix.starts[h], s), ce = min((int64_t)ix.ends[h], e);
+        if (cs > cursor) {
+            if (FILL) {
+                if (os) os[w] = (S)cursor;
+                if (oe) oe[w] = (S)(cs - 1);
+                if (od) od[w] = d;
+            }
+            ++w;
+        }
+        if (ce >= e) {
+            past = true;
+            break;
+        }
+        if (ce + 1 > cursor) cursor = ce + 1;
+    }
+    if (!past && cursor <= e) {
+        if (FILL) {
+            if (os) os[w] = (S)cursor;
+            if (oe) oe[w] = (S)e;
+            if (od) od[w] = d;
+        }
+        ++w;
+    }
+    if (!FILL) pieces[k] = w;
+}
+
+SI_EXPORT int siIntersectPieces(const siIndex* ix, const void* as, const void* ae, const int32_t* ad, size_t na,
+                                const uint64_t* offsets, const uint32_t* idx, void* out_starts, void* out_ends,
+                                int32_t* out_adata, int32_t* out_bdata, void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (na == 0) return SI_OK;
+    if (!as || !ae || !offsets) return si_set_error(SI_ERR_ARG, "null buffer");
+    if (out_bdata && !ix->d_values) return si_set_error(SI_ERR_ARG, "index was built without values");
+    hipStream_t st = (hipStream_t)stream;
+    return DISPATCH_S(ix, ([&]() -> int {
+        k_intersect_pieces<S><<<grid_for(na, Q_THREADS), Q_THREADS, 0, st>>>(
+            ix->view<S>(), ix->d_values, (const S*)as, (const S*)ae, ad, na, offsets, idx, (S*)out_starts,
+            (S*)out_ends, out_adata, out_bdata);
+        LAUNCH_CHECK();
+        return SI_OK;
+    }()));
+}
+
+SI_EXPORT int siDifferenceOffsets(const siIndex* ix, const void* as, const void* ae, size_t na,
+                                  const uint64_t* offsets, const uint32_t* idx, uint64_t* piece_offsets,
+                                  void* workspace, size_t workspace_bytes, void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (!piece_offsets) return si_set_error(SI_ERR_ARG, "null piece offsets");
+    if (na && (!as || !ae || !offsets)) return si_set_error(SI_ERR_ARG, "null buffer");
+    if (!workspace || workspace_bytes < siSearchWorkspaceBytes(na))
+        return si_set_error(SI_ERR_ARG, "workspace too small: %zu < %zu bytes", workspace_bytes,
+                            siSearchWorkspaceBytes(na));
+    hipStream_t st = (hipStream_t)stream;
+    if (na) {
+        rc = DISPATCH_S(ix, ([&]() -> int {
+            k_difference_pieces<S, false><<<grid_for(na, Q_THREADS), Q_THREADS, 0, st>>>(
+                ix->view<S>(), (const S*)as, (const S*)ae, nullptr, na, offsets, idx, piece_offsets, nullptr,
+                nullptr, nullptr);
+            LAUNCH_CHECK();
+            return SI_OK;
+        }()));
+        if (rc) return rc;
+    }
+    return device_excl_scan<uint64_t, uint64_t>(piece_offsets, na, piece_offsets, piece_offsets + na,
+                                                (uint64_t*)workspace, st);
+}
+
+SI_EXPORT int siDifferenceFill(const siIndex* ix, const void* as, const void* ae, const int32_t* ad, size_t na,
+                               const uint64_t* offsets, const uint32_t* idx, const uint64_t* piece_offsets,
+                               void* out_starts, void* out_ends, int32_t* out_data, void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (na == 0) return SI_OK;
+    if (!as || !ae || !offsets || !piece_offsets) return si_set_error(SI_ERR_ARG, "null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    return DISPATCH_S(ix, ([&]() -> int {
+        k_difference_pieces<S, true><<<grid_for(na, Q_THREADS), Q_THREADS, 0, st>>>(
+            ix->view<S>(), (const S*)as, (const S*)ae, ad, na, offsets, idx, (uint64_t*)piece_offsets,
+            (S*)out_starts, (S*)out_ends, out_data);
+        LAUNCH_CHECK();
+        return SI_OK;
+    }()));
+}
+
 SI_EXPORT int siCoverage(const siIndex* ix, const void* qs, const void* qe, size_t nq, uint32_t* count_out,
                          int32_t* cov_out, void* stream) {
     int rc = check_ix(ix);
