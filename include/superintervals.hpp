@@ -263,6 +263,58 @@ class IntervalMap {
         return true;
     }
 
+    // ---- set operations (superintervals.hpp:1158-1211) ---------------------------------
+    // Both run one batched search of this set's intervals (in their current order) in
+    // other's index on the GPU, then assemble the pieces with this set's / other's data on
+    // the host (T is any host type).  other must be built; the results are unindexed.
+    //
+    // superintervals.hpp:1158-1174: every overlapping piece, combine(this data, other data).
+    template <typename Combine>
+    IntervalMap<S, T> intersection(const IntervalMap<S, T>& other, Combine combine) const {
+        IntervalMap<S, T> out;
+        std::vector<uint64_t> off;
+        std::vector<uint32_t> idx;
+        if (!other.search_idxs_batch(starts.data(), ends.data(), starts.size(), off, idx, true)) return out;
+        for (size_t k = 0; k < starts.size(); ++k) {
+            for (uint64_t j = off[k]; j < off[k + 1]; ++j) {
+                const size_t h = idx[j];
+                const S s = starts[k] > other.starts[h] ? starts[k] : other.starts[h];
+                const S e = ends[k] < other.ends[h] ? ends[k] : other.ends[h];
+                if (s <= e) out.add(s, e, combine(data[k], other.data[h]));
+            }
+        }
+        return out;
+    }
+    IntervalMap<S, T> intersection(const IntervalMap<S, T>& other) const {
+        return intersection(other, [](const T& a, const T&) { return a; });
+    }
+    // superintervals.hpp:1188-1211: the parts of each interval that other does not cover.
+    // other's hits in descending position order, read backwards, are its keys sorted by
+    // (start, end).  The sweep stops once a hit covers the interval to its end.
+    IntervalMap<S, T> difference(const IntervalMap<S, T>& other) const {
+        IntervalMap<S, T> out;
+        std::vector<uint64_t> off;
+        std::vector<uint32_t> idx;
+        if (!other.search_idxs_batch(starts.data(), ends.data(), starts.size(), off, idx, false)) return out;
+        for (size_t k = 0; k < starts.size(); ++k) {
+            S cursor = starts[k];
+            bool past = false;
+            for (uint64_t j = off[k + 1]; j-- > off[k];) {
+                const size_t h = idx[j];
+                const S cs = other.starts[h] > starts[k] ? other.starts[h] : starts[k];
+                const S ce = other.ends[h] < ends[k] ? other.ends[h] : ends[k];
+                if (cs > cursor) out.add(cursor, cs - 1, data[k]);
+                if (ce >= ends[k]) {
+                    past = true;
+                    break;
+                }
+                if (ce + 1 > cursor) cursor = ce + 1;
+            }
+            if (!past && cursor <= ends[k]) out.add(cursor, ends[k], data[k]);
+        }
+        return out;
+    }
+
     // Device-resident index (valid after build()); for the device entry points.
     const siIndex* device_index() const { return dev_; }
     static std::string last_error() { return siGetLastError(); }

@@ -187,6 +187,42 @@ static void randomized(uint64_t seed) {
     }
 }
 
+// intersection / difference (superintervals.hpp:1158-1211) on a hand-checked case
+void set_ops() {
+    si::IntervalMap<int32_t, int> a, b;
+    a.add(1, 100, 0); a.add(40, 60, 1); a.add(200, 210, 2); a.add(95, 130, 3);   // not built
+    b.add(10, 20, 10); b.add(50, 70, 11); b.add(55, 58, 12); b.add(205, 400, 13);
+    b.build();
+    si::IntervalMap<int32_t, int> d = a.difference(b);
+    const int want[][3] = {{1, 9, 0}, {21, 49, 0}, {71, 100, 0}, {40, 49, 1}, {200, 204, 2}, {95, 130, 3}};
+    CHECK(d.size() == 6);
+    for (size_t i = 0; i < 6 && i < d.size(); ++i)
+        CHECK(d.starts[i] == want[i][0] && d.ends[i] == want[i][1] && d.data[i] == want[i][2]);
+    si::IntervalMap<int32_t, int> x = a.intersection(b, [](const int& p, const int& q) { return p * 100 + q; });
+    // pieces per a interval, other's hits in search_idxs order
+    size_t n = 0;
+    for (size_t k = 0; k < a.starts.size(); ++k) {
+        std::vector<size_t> h;
+        b.search_idxs(a.starts[k], a.ends[k], h);
+        for (size_t j : h) {
+            CHECK(n < x.size());
+            if (n >= x.size()) break;
+            CHECK(x.starts[n] == std::max(a.starts[k], b.starts[j]) && x.ends[n] == std::min(a.ends[k], b.ends[j]));
+            CHECK(x.data[n] == a.data[k] * 100 + b.data[j]);
+            ++n;
+        }
+    }
+    CHECK(n == x.size() && n == 6);   // [10,20] [50,70] [55,58] in a0; [50,60] [55,58] in a1; [205,210] in a2
+    // the type's maximum: covered to the end leaves only the lower part
+    si::IntervalMap<int32_t, int> c, e;
+    const int32_t big = 2147483647;
+    c.add(big - 5, big, 7);
+    e.add(big - 2, big, 8);
+    e.build();
+    si::IntervalMap<int32_t, int> cd = c.difference(e);
+    CHECK(cd.size() == 1 && cd.starts[0] == big - 5 && cd.ends[0] == big - 3 && cd.data[0] == 7);
+}
+
 int main() {
     siIndex* probe = siIndexCreate(4);
     if (!probe) {
@@ -197,6 +233,7 @@ int main() {
     known_answers();
     randomized<int32_t>(1);
     randomized<int64_t>(2);
+    set_ops();
     if (failures) {
         std::fprintf(stderr, "%d failure(s)\n", failures);
         return 1;
