@@ -567,6 +567,8 @@ def main():
     t_plan = sum(ev[0].elapsed_time(ev[1]) for ev in evs) / a.steps
     t_off = sum(ev[1].elapsed_time(ev[2]) for ev in evs) / a.steps
     t_fill = sum(ev[2].elapsed_time(ev[3]) for ev in evs) / a.steps
+    # batch latency: plan -> offsets -> fill of one batch of nq queries, per timed step
+    lat_ms = np.array([ev[0].elapsed_time(ev[3]) for ev in evs])
     ms_step = wall * 1e3 / a.steps
 
     # count walk alone on the planned batch (one launch per pass); the walk form is forced
@@ -639,6 +641,8 @@ def main():
         "roofline": roof,
         "kernels": kernels,
         # count() as tuned (walk or rank form), and the walk alone
+        "batch_latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99)),
+                             "batch_queries": nq, "batches": int(lat_ms.size)},
         "count_queries_per_s": nq / ((t_count_rank if tuned_count_rank else t_count) / 1e3),
         "count_walk_queries_per_s": nq / (t_count / 1e3),
         "walk": {"hits": hits, "ends_reads": V, "branch_reads": J, "found": found},
