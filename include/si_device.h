@@ -115,8 +115,8 @@ double siIndexLastBuildMs(const siIndex* ix);
  *   "tile"       sorted queries per workgroup tile (64 .. 2^20, default 1024)
  *   "count_wgs"  resident workgroups per CU in the count pass (0 = occupancy limit)
  *   "fill_wgs"   resident workgroups per CU in the fill passes (0 = occupancy limit)
- *   "count_rank"   1: siCount* use the rank form (upper_bound + 1 - #{ends < s}, exact
- *                  for s <= e; inverted queries walk) instead of the walk (default 0)
+ *   "count_rank"   1 (default): siCount* use the rank form (upper_bound + 1 - #{ends < s},
+ *                  exact for s <= e; inverted queries walk); 0: the walk
  *   "offsets_rank" 1 (default): siSearchOffsets* count with the rank form; 0: the walk
  *   "lane_steps"   line steps a lane spends on one query before the walk is parked for
  *                  the wave-cooperative kernel (1 .. 2^30, default 256)

@@ -204,7 +204,7 @@ __global__ __launch_bounds__(Q_THREADS) void k_count_rank(IndexView<S> ix, const
 //   tile         sorted queries per workgroup tile
 //   count_wgs    resident workgroups per CU for the count pass (0 = occupancy limit)
 //   fill_wgs     resident workgroups per CU for the fill passes (0 = occupancy limit)
-//   count_rank   siCount / siCountPlanned use the rank form instead of the walk
+//   count_rank   siCount / siCountPlanned use the rank form (default) instead of the walk
 //   offsets_rank siSearchOffsets* use the rank form (default) instead of the walk
 //   lane_steps   line steps a lane spends on one query before parking it for the
 //                wave-cooperative kernel
@@ -223,7 +223,7 @@ struct Tuning {
     uint32_t tile = 1024;
     int count_wgs = 0;
     int fill_wgs = 0;
-    int count_rank = 0;
+    int count_rank = 1;
     int offsets_rank = 1;
     uint32_t lane_steps = 256;
     int plan_sync = 1;

@@ -103,13 +103,14 @@ def test_random_vs_oracle(sia, orc, seed, dtype):
     assert np.array_equal(a["branch"], oix["branch"])
     assert np.array_equal(a["values"], oix["data"])
     ocount = orc.count_batch(oix, qs, qe)
-    assert np.array_equal(ix.count(qs, qe), ocount)
     from superintervals_amd import autotune
-    autotune.apply({"count_rank": 1})       # rank form, walk fallback for inverted queries
+    saved = autotune.current()
     try:
-        assert np.array_equal(ix.count(qs, qe), ocount)
+        for cr in (0, 1):                   # the walk, and the rank form (walk for inverted queries)
+            autotune.apply({"count_rank": cr})
+            assert np.array_equal(ix.count(qs, qe), ocount), cr
     finally:
-        autotune.apply({"count_rank": 0})
+        autotune.apply(saved)
     for order, mode in (("desc", 0), ("vector", 1)):
         off, idx = ix.search(qs, qe, order=order)
         ooff, oidx = orc.search_batch(oix, qs, qe, mode=mode)
