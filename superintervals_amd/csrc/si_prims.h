@@ -154,28 +154,45 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(TO* __restrict__ ts, size_t
 }
 
 // Downsweep: out[i] = tile prefix + exclusive scan within the tile.  Each thread owns
-// SCAN_ITEMS consecutive items.  in and out may alias.
+// SCAN_ITEMS consecutive items; the tile passes through LDS (one pad entry per 32 items)
+// so that global loads and stores are coalesced across the wave.  A block reads and
+// writes only its own tile, all reads before any write, so in and out may alias.
 template <typename TI, typename TO>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const TI* in, size_t n,
                                                             const TO* __restrict__ tile_pre,
                                                             TO* out) {
     __shared__ TO s_w[4];
-    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    __shared__ TO s_v[SCAN_TILE + SCAN_TILE / 32];
+    const size_t tbase = (size_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        const uint32_t li = (uint32_t)k * SCAN_THREADS + threadIdx.x;
+        const size_t i = tbase + li;
+        s_v[li + li / 32] = (i < n) ? (TO)in[i] : (TO)0;
+    }
+    __syncthreads();
     TO v[SCAN_ITEMS];
     TO sum = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
-        size_t i = base + k;
-        v[k] = (i < n) ? (TO)in[i] : (TO)0;
+        const uint32_t li = threadIdx.x * SCAN_ITEMS + k;
+        v[k] = s_v[li + li / 32];
         sum += v[k];
     }
     TO tot;
     TO run = block_excl_scan256<TO>(sum, s_w, &tot) + tile_pre[blockIdx.x];
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
-        size_t i = base + k;
-        if (i < n) out[i] = run;
+        const uint32_t li = threadIdx.x * SCAN_ITEMS + k;
+        s_v[li + li / 32] = run;
         run += v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        const uint32_t li = (uint32_t)k * SCAN_THREADS + threadIdx.x;
+        const size_t i = tbase + li;
+        if (i < n) out[i] = s_v[li + li / 32];
     }
 }
 
