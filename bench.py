@@ -152,7 +152,7 @@ def bench_learned(a):
 
 def load_traffic():
     """HBM bytes per launch of the walk kernels from the committed PMC summary, if any."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_workload2.json")
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_search_lognormal_10Mx10M.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:

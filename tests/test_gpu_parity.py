@@ -648,11 +648,11 @@ def test_cpp_api_program(sia):
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
 
 
-def test_full_size_workload2_properties(sia, orc):
-    """10M intervals x 10M queries (workload 2): size-independent properties of the whole
+def test_full_size_lognormal_10Mx10M_properties(sia, orc):
+    """10M intervals x 10M queries (log-normal lengths): size-independent properties of the whole
     output plus an oracle check of a query sample."""
     import torch
-    print("[gpu] full-size workload 2", flush=True)
+    print("[gpu] full-size log-normal 10M x 10M", flush=True)
     n = nq = 10_000_000
     s, e = sia.workload_intervals(2, n, seed=1, sorted=True)
     qs, qe = sia.workload_queries(2, nq, seed=2)
