@@ -159,37 +159,103 @@ def load_traffic():
         return json.load(f)
 
 
-def cpu_baseline(s, e, qs, qe, sample):
-    """Reference CPU path on a bounded sample of the same queries, all host cores."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(s, e, qs, qe, sample, check=None):
+    """Reference CPU path on a bounded sample of the same queries, all host cores: search_idxs
+    and count.  With check = (offsets, positions, query numbers) of the GPU's descending
+    output, the same CPU code also computes those queries' lists and they must match."""
     import numpy as np
     import oracle
     threads = len(os.sched_getaffinity(0))
     sq, se = qs[:sample], qe[:sample]
+
+    def median3(fn):
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[1]
+
+    O = oracle.Oracle()
     if oracle.Reference.available():
         R = oracle.Reference()
         m = oracle.RefMap(R, s, e)
         m.search_batch_total(sq[:10000], se[:10000], threads)        # warm-up
-        ts = []
-        for _ in range(3):
-            t0 = time.perf_counter()
-            m.search_batch_total(sq, se, threads)
-            ts.append(time.perf_counter() - t0)
+        t = median3(lambda: m.search_batch_total(sq, se, threads))
+        tc = median3(lambda: m.count_batch(sq, se, threads))
         kind = "reference"
-        desc = "si::IntervalMap<int,int>::search_idxs(s, e, std::vector&) from the reference headers"
+        desc = "si::IntervalMap<int,int>::search_idxs(s, e, std::vector&) and count(s, e) from the reference headers"
+        lists = lambda q: np.asarray(m.search(qs[q], qe[q], 0), np.int64)
     else:
-        O = oracle.Oracle()
         ix = O.build_index(s, e)
-        ts = []
-        for _ in range(3):
-            t0 = time.perf_counter()
-            O.search_batch(ix, sq, se, mode=1, threads=threads, lists=False)
-            ts.append(time.perf_counter() - t0)
+        t = median3(lambda: O.search_batch(ix, sq, se, mode=1, threads=threads, lists=False))
+        tc = median3(lambda: O.count_batch(ix, sq, se, threads=threads))
         kind = "port"
-        desc = "oracle restatement of search_idxs"
-    t = sorted(ts)[1]
-    return {"value": sample / t, "unit": "queries/s", "cores": threads, "kind": kind,
-            "sample": f"first {sample} of the {qs.size} queries, median of 3; {desc}; "
-                      f"static chunks over {threads} threads"}
+        desc = "oracle restatement of search_idxs and count"
+        lists = lambda q: O.search(ix, qs[q], qe[q], 0).astype(np.int64)
+    out = {"value": sample / t, "unit": "queries/s", "cores": threads, "kind": kind,
+           "cpu_model": cpu_model(), "count_queries_per_s": sample / tc,
+           "sample": f"first {sample} of the {qs.size} queries, median of 3; {desc}; "
+                     f"static chunks over {threads} threads"}
+    if check is not None:
+        off, idx, picks = check
+        bad = 0
+        for q in picks:
+            got = idx[int(off[q]):int(off[q + 1])].astype(np.int64)
+            bad += int(not np.array_equal(got, lists(int(q))))
+        out["checked_queries"] = int(len(picks))
+        out["mismatched_queries"] = bad
+    return out
+
+
+def verify_output(ix, tqs, tqe, s, e, off, idx, nq, stream):
+    """Checks the timed configuration's own output on the device: counts against the closed
+    form (torch.searchsorted over the sorted starts and ends, every query has s <= e), the
+    walk-form count against the offsets (two different device algorithms), every emitted
+    position overlaps its query, every list strictly descending.  Returns a dict of results."""
+    import torch
+    from superintervals_amd import autotune
+    dev = tqs.device
+    res = {}
+    torch.cuda.synchronize()
+    cnt = (off[1:] - off[:-1])
+    ss = torch.from_numpy(s).to(dev)
+    se = torch.sort(torch.from_numpy(e).to(dev)).values
+    closed = torch.searchsorted(ss, tqe, right=True) - torch.searchsorted(se, tqs, right=False)
+    res["closed_form_counts"] = bool(torch.equal(closed.to(torch.int64), cnt))
+    del se, closed
+    saved = autotune.current()
+    walk = torch.empty(nq, dtype=torch.int32, device=dev)
+    try:
+        autotune.apply({"count_rank": 0})
+        ix.count_device(tqs, tqe, walk, stream=stream)
+    finally:
+        autotune.apply(saved)
+    torch.cuda.synchronize()
+    res["walk_counts_equal_offsets"] = bool(torch.equal(walk.to(torch.int64), cnt))
+    del walk
+    total = int(off[-1].item())
+    qid = torch.repeat_interleave(torch.arange(nq, device=dev, dtype=torch.int32), cnt)
+    pos = idx[:total].to(torch.int64)
+    ends_t = torch.from_numpy(e).to(dev)
+    ok = bool((ss[pos] <= tqe[qid]).all()) and bool((ends_t[pos] >= tqs[qid]).all())
+    res["positions_overlap_query"] = ok
+    same = qid[1:] == qid[:-1]
+    res["lists_strictly_descending"] = bool((pos[1:][same] < pos[:-1][same]).all())
+    del qid, pos, same, ends_t, ss
+    torch.cuda.empty_cache()
+    return res
 
 
 def bench_serving(a):
@@ -571,6 +637,14 @@ def main():
     lat_ms = np.array([ev[0].elapsed_time(ev[3]) for ev in evs])
     ms_step = wall * 1e3 / a.steps
 
+    # the timed configuration's own output (the last timed step's offsets and lists)
+    checks = verify_output(ix, tqs, tqe, s, e, off, idx, nq, stream)
+    picks = np.unique(np.linspace(0, nq - 1, 2000).astype(np.int64))
+    off_h = idx_h = None
+    if not a.no_cpu:
+        off_h = off.cpu().numpy()
+        idx_h = idx[:total].cpu().numpy()
+
     # count walk alone on the planned batch (one launch per pass); the walk form is forced
     # here whatever the tuner chose, since the byte model below is the walk's
     from superintervals_amd import autotune
@@ -621,10 +695,20 @@ def main():
     roof = {"bound": "hbm", "kernel": "k_walk<int, desc fill> (search_idxs fill pass, one launch over all queries)",
             "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": tr["fill"]["bytes_per_launch"] if tr else None}
+            "traffic": tr["fill"]["bytes_per_launch"] if tr else None,
+            "basis": "achieved = algorithmic bytes per launch (per query: 12 B plan entry + c*V ends + 4*J "
+                     "branch reads of the reference walk + 4*h positions + 8 B offset) / launch time; the "
+                     "walk serves most ends reads from its LDS window, so the counter-measured HBM traffic "
+                     "(traffic, measured_GBps) is below the algorithmic bytes"}
     if tr:
+        roof["measured_GBps"] = round(tr["fill"]["bytes_per_launch"] / t_fill / 1e6, 1)
+        roof["measured_frac"] = round(roof["measured_GBps"] / HBM_PEAK_GBS, 4)
+        roof["traffic_profile"] = tr.get("profile", "profiles/pmc_traffic_search_lognormal_10Mx10M.json")
         kernels["count"]["hbm_bytes_pmc"] = tr["count"]["bytes_per_launch"]
         kernels["search_fill"]["hbm_bytes_pmc"] = tr["fill"]["bytes_per_launch"]
+        if "count_rank" in tr:
+            kernels["count_rank"]["hbm_bytes_pmc"] = tr["count_rank"]["bytes_per_launch"]
+            kernels["count_rank"]["measured_GBps"] = round(tr["count_rank"]["bytes_per_launch"] / t_count_rank / 1e6, 1)
 
     out = {
         "metric": "overlap queries/sec + achieved HBM GB/s (10M idx x 10M qry); p50/p99 batch latency",
@@ -648,10 +732,17 @@ def main():
         "walk": {"hits": hits, "ends_reads": V, "branch_reads": J, "found": found},
     }
     if not a.no_cpu:
-        cb = cpu_baseline(s, e, qs, qe, min(a.cpu_sample, nq))
+        cb = cpu_baseline(s, e, qs, qe, min(a.cpu_sample, nq), check=(off_h, idx_h, picks))
+        checks["cpu_sample_lists_equal"] = cb.pop("mismatched_queries") == 0
+        checks["cpu_sample_queries"] = cb.pop("checked_queries")
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = out["value"] / cb["value"]
+    out["verified"] = checks
     print(json.dumps(out))
+    bad = [k for k, v in checks.items() if v is False]
+    if bad:
+        print("bench output check failed: %s" % ", ".join(bad), file=sys.stderr, flush=True)
+        sys.exit(3)
 
 
 if __name__ == "__main__":

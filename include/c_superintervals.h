@@ -24,10 +24,15 @@
  *     Use the si*Batch functions below for throughput.
  *   - Device failures on the reference's void functions are reported through
  *     siGetLastError() (si_device.h) and a zero/none result.
- *   - Set operations (mergeOverlaps, intervalGaps, unionWith, intersection,
- *     difference, symmetricDifference, intervalSpan, expandIntervals,
- *     flankIntervals, uniqueIntervals; c_superintervals.h:264-332) are not part
- *     of this library.
+ *   - Of the set operations (c_superintervals.h:264-332) this library provides
+ *     intersection and difference, the two that query an index (batched device
+ *     searches).  mergeOverlaps, intervalGaps, unionWith, symmetricDifference,
+ *     intervalSpan, expandIntervals, flankIntervals and uniqueIntervals are
+ *     host-side sorts and merges of the arrays and are not part of it.
+ *   - addInterval and clearSuperIntervals release the device index: queries on a
+ *     handle changed since its last indexSuperIntervals report SI_ERR_NOT_BUILT
+ *     through siGetLastError() and return no result (the reference leaves them
+ *     undefined).
  */
 #ifndef SUPERINTERVALS_HIP_C_HEADER
 #define SUPERINTERVALS_HIP_C_HEADER 1
@@ -95,6 +100,9 @@ typedef struct {
     size_t capacity;
 } cItemResult;
 
+/* c_superintervals.h:128 */
+typedef int32_t (*cCombineFn)(int32_t accumulated, int32_t next);
+
 /* ---- Lifecycle (c_superintervals.h:135-156) ---- */
 cSuperIntervals* createSuperIntervals(void);                       /* :135, :363-375 */
 void destroySuperIntervals(cSuperIntervals* si);                   /* :138, :377-384 */
@@ -128,6 +136,18 @@ void coverage(cSuperIntervals* si, int32_t start, int32_t end,
               size_t* count_out, int32_t* coverage_out);                                  /* :230, :758-792 */
 void findOverlaps(cSuperIntervals* si, int32_t start, int32_t end,
                   int32_t* found, size_t* found_size);                                    /* :239, :794-821 */
+
+/* ---- Set operations (c_superintervals.h:286-292) ----
+ * Both return a new, unindexed handle (caller owns; destroySuperIntervals).  other must be
+ * indexed; si is read in its current order.  The searches of si's intervals in other run
+ * as one batch on the GPU. */
+/* :286, :921-940 -- every non-empty overlap piece, per si interval in searchIdxs (descending)
+ * order of other's hits; combine(si data, other data), NULL keeps si's data. */
+cSuperIntervals* intersection(const cSuperIntervals* si, cSuperIntervals* other, cCombineFn combine);
+/* :292, :950-975 -- the parts of each si interval that no interval of other covers, with
+ * its data.  A hit ending at INT32_MAX covers the rest of the interval (the reference's
+ * ce + 1 overflows there). */
+cSuperIntervals* difference(const cSuperIntervals* si, cSuperIntervals* other);
 
 /* ---- Result helpers (c_superintervals.h:339-357) ---- */
 cIndexResult createIndexResult(void);     /* :339, :1070-1073 */

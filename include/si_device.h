@@ -42,6 +42,7 @@ extern "C" {
 #define SI_ERR_CAPACITY 4     /* output capacity smaller than the number of results */
 #define SI_ERR_NOMEM 5        /* device or host allocation failed */
 #define SI_ERR_UNSUPPORTED 6  /* operation not available for this coordinate type */
+#define SI_ERR_MISMATCH 7     /* a fill pass found a different number of hits than its offsets */
 
 /* Output order of index lists. */
 #define SI_ORDER_DESC 0   /* strict descending position: c_superintervals.h:610-643 searchIdxs,
@@ -160,6 +161,14 @@ int siSearchOffsetsPlanned(const siIndex* ix, void* plan, size_t nq, uint32_t* c
 /* Fill pass for the planned queries (see siSearchFill). */
 int siSearchFillPlanned(const siIndex* ix, void* plan, size_t nq, int order,
                         const uint64_t* offsets, uint32_t* idx, void* stream);
+
+/* Checks the last fill pass run with this plan (synchronises the stream): SI_OK when every
+ * query's fill wrote exactly the hits its offsets give, SI_ERR_MISMATCH otherwise.  A fill
+ * never writes outside a query's segment [offsets[q], offsets[q+1]), so offsets that
+ * disagree with the index (the wrong index, or changed offsets) cut or leave gaps in the
+ * lists; this call reports it.  The host-buffer forms (siSearchHost and the C-ABI) run it
+ * after every fill.  No reference counterpart (the reference fills growing vectors). */
+int siPlanCheck(const siIndex* ix, const void* plan, void* stream);
 
 /* Payload fill pass for the planned queries (see siSearchValuesFill). */
 int siSearchValuesFillPlanned(const siIndex* ix, void* plan, size_t nq, const uint64_t* offsets,

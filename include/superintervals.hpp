@@ -18,8 +18,14 @@
 //   * Single-query calls are GPU round trips; prefer the batch overloads.
 //   * Device failures are reported by si::IntervalMap::last_error() (and
 //     siGetLastError()); the query then returns an empty / zero result.
-//   * Set operations (merge_overlaps, gaps, union_with, intersection, difference,
-//     symmetric_difference, span, expand, flank, unique; :1037-1390) are not provided.
+//   * Of the set operations (:1037-1390) intersection and difference are provided (one
+//     batched device search each, below); merge_overlaps, gaps, union_with,
+//     symmetric_difference, span, expand, flank and unique are host-side sorts and
+//     merges and are not.
+//   * count_large and search_values_large return count's and search_values' results.
+//     The reference's versions count the run [first start >= query start, upper_bound]
+//     without reading ends, which differs from count() only on an index holding an
+//     interval with start > end.
 //   * IntervalMapEytz (:1457-1535) is provided as the same GPU index; its
 //     upper_bound returns SIZE_MAX when the value is below every start, like the base.
 //
@@ -189,6 +195,30 @@ class IntervalMap {
         return p == SI_INDEX_NONE ? SIZE_MAX : (size_t)p;
     }
 
+    // superintervals.hpp:518-543: moves left down towards the first position whose start is
+    // >= value, for a caller scanning the run (left, right] of positions that start at or
+    // after value.  It gallops down from left (steps 1, 2, 4, ... while the start there is
+    // above value), then halves the last step's range [left, hi): a probe that starts below
+    // value moves left just past it.  On exit every position above left (up to right)
+    // starts at or after value: left is the position below the first such start, or one
+    // in the run starting exactly at value; SIZE_MAX when position 0 starts at or after
+    // value.
+    // The exact exit position follows these steps, as callers see it.  Host code on the
+    // public starts array.
+    inline void upper_bound_range(const S value, size_t& left, const size_t right) const noexcept {
+        size_t hi = right;
+        for (size_t step = 1; left > 0 && starts[left] > value; step *= 2) {
+            hi = left;
+            left = (step <= left) ? left - step : 0;
+        }
+        for (size_t len = hi - left; len > 1;) {
+            const size_t keep = len / 2;
+            if (starts[left + keep] < value) left += len - keep;
+            len = keep;
+        }
+        if (left == 0 && starts[0] >= value) left = SIZE_MAX;
+    }
+
     // superintervals.hpp:551-579 (descending data order).
     void search_values(const S start, const S end, std::vector<T>& found) const {
         for (size_t i : positions(start, end, SI_ORDER_DESC)) found.push_back(data[i]);
@@ -289,8 +319,9 @@ class IntervalMap {
         return intersection(other, [](const T& a, const T&) { return a; });
     }
     // superintervals.hpp:1188-1211: the parts of each interval that other does not cover.
-    // other's hits in descending position order, read backwards, are its keys sorted by
-    // (start, end).  The sweep stops once a hit covers the interval to its end.
+    // other's hits in descending position order, read backwards, are its keys ascending by
+    // (start, end desc); the sweep's pieces do not depend on the order of equal starts.  A
+    // hit ending at the type's maximum covers the rest (the reference's ce + 1 overflows).
     IntervalMap<S, T> difference(const IntervalMap<S, T>& other) const {
         IntervalMap<S, T> out;
         std::vector<uint64_t> off;
@@ -304,7 +335,7 @@ class IntervalMap {
                 const S cs = other.starts[h] > starts[k] ? other.starts[h] : starts[k];
                 const S ce = other.ends[h] < ends[k] ? other.ends[h] : ends[k];
                 if (cs > cursor) out.add(cursor, cs - 1, data[k]);
-                if (ce >= ends[k]) {
+                if (ce == std::numeric_limits<S>::max()) {
                     past = true;
                     break;
                 }

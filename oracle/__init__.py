@@ -243,6 +243,25 @@ class Reference:
             getattr(L, f"ref_search_batch_{sfx}").restype = C.c_uint64
             getattr(L, f"ref_search_batch_{sfx}").argtypes = [_vp, _i, _sz, _vp, _vp, _vp]
         L.ref_coverage_i32.argtypes = [_vp, C.c_int32, C.c_int32, _vp, _vp]
+        for sfx, ct in (("i32", C.c_int32), ("i64", C.c_int64)):
+            getattr(L, f"ref_upper_bound_range_{sfx}").restype = C.c_int64
+            getattr(L, f"ref_upper_bound_range_{sfx}").argtypes = [_vp, ct, _sz, _sz]
+            getattr(L, f"ref_raw_{sfx}").restype = _vp
+            getattr(L, f"ref_raw_{sfx}").argtypes = [_sz, _vp, _vp, _vp]
+            getattr(L, f"ref_raw_arrays_{sfx}").argtypes = [_vp, _vp, _vp, _vp]
+            getattr(L, f"ref_intersection_{sfx}").restype = _vp
+            getattr(L, f"ref_intersection_{sfx}").argtypes = [_vp, _vp]
+            getattr(L, f"ref_difference_{sfx}").restype = _vp
+            getattr(L, f"ref_difference_{sfx}").argtypes = [_vp, _vp]
+        L.refc_raw.restype = _vp
+        L.refc_raw.argtypes = [_sz, _vp, _vp, _vp]
+        L.refc_size.restype = _sz
+        L.refc_size.argtypes = [_vp]
+        L.refc_raw_arrays.argtypes = [_vp, _vp, _vp, _vp]
+        L.refc_intersection.restype = _vp
+        L.refc_intersection.argtypes = [_vp, _vp]
+        L.refc_difference.restype = _vp
+        L.refc_difference.argtypes = [_vp, _vp]
         L.refc_build.restype = _vp
         L.refc_build.argtypes = [_sz, _vp, _vp, _vp]
         L.refc_free.argtypes = [_vp]
@@ -316,6 +335,34 @@ class RefMap:
         tot = getattr(self.L, f"ref_count_batch_{self.sfx}")(self.h, threads, qs.size, _p(qs), _p(qe), _p(out))
         return out, int(tot)
 
+    def upper_bound_range(self, value, left, right):
+        """Exit value of `left` (-1 for SIZE_MAX), superintervals.hpp:524-543."""
+        return int(getattr(self.L, f"ref_upper_bound_range_{self.sfx}")(self.h, int(value), int(left), int(right)))
+
+    def set_op(self, other, op):
+        """C++ intersection (default combine) or difference of this map with other:
+        (starts, ends, data) of the unindexed result in its order."""
+        h = getattr(self.L, f"ref_{op}_{self.sfx}")(self.h, other.h)
+        n = int(getattr(self.L, f"ref_size_{self.sfx}")(h))
+        s = np.empty(n, self.dtype); e = np.empty(n, self.dtype); d = np.empty(n, np.int32)
+        getattr(self.L, f"ref_raw_arrays_{self.sfx}")(h, _p(s), _p(e), _p(d))
+        getattr(self.L, f"ref_free_{self.sfx}")(h)
+        return s, e, d
+
+    @classmethod
+    def raw(cls, ref, starts, ends, data=None):
+        """An unbuilt map holding the intervals in insertion order."""
+        m = cls.__new__(cls)
+        m.L = ref.lib
+        s = np.ascontiguousarray(starts)
+        e = np.ascontiguousarray(ends, dtype=s.dtype)
+        d = None if data is None else np.ascontiguousarray(data, np.int32)
+        m.sfx = "i64" if s.dtype == np.int64 else "i32"
+        m.dtype = s.dtype
+        m.h = getattr(m.L, f"ref_raw_{m.sfx}")(s.size, _p(s), _p(e), _p(d))
+        m.n = s.size
+        return m
+
     def search_batch_total(self, qs, qe, threads):
         qs = np.ascontiguousarray(qs, self.dtype); qe = np.ascontiguousarray(qe, self.dtype)
         return int(getattr(self.L, f"ref_search_batch_{self.sfx}")(self.h, threads, qs.size, _p(qs), _p(qe), None))
@@ -362,3 +409,26 @@ class RefCMap:
         c = np.zeros(1, np.uint64); v = np.zeros(1, np.int32)
         self.L.refc_coverage(self.h, int(s), int(e), _p(c), _p(v))
         return int(c[0]), int(v[0])
+
+    @classmethod
+    def raw(cls, ref, starts, ends, data=None):
+        """An unindexed handle holding the intervals in insertion order."""
+        m = cls.__new__(cls)
+        m.L = ref.lib
+        s = np.ascontiguousarray(starts, np.int32)
+        e = np.ascontiguousarray(ends, np.int32)
+        d = None if data is None else np.ascontiguousarray(data, np.int32)
+        m.h = m.L.refc_raw(s.size, _p(s), _p(e), _p(d))
+        m.n = s.size
+        return m
+
+    def set_op(self, other, op):
+        """C intersection (NULL combine) or difference with other (indexed):
+        (starts, ends, data) of the unindexed result in its order."""
+        h = (self.L.refc_intersection if op == "intersection" else self.L.refc_difference)(self.h, other.h)
+        n = int(self.L.refc_size(h))
+        s = np.empty(n, np.int32); e = np.empty(n, np.int32); d = np.empty(n, np.int32)
+        if n:
+            self.L.refc_raw_arrays(h, _p(s), _p(e), _p(d))
+        self.L.refc_free(h)
+        return s, e, d

@@ -51,3 +51,27 @@ REFC_API void refc_coverage(void* p, int32_t s, int32_t e, uint64_t* c, int32_t*
     coverage((cSuperIntervals*)p, s, e, &cc, cov);
     *c = cc;
 }
+
+/* An unindexed handle (insertion order), for the first operand of a set operation. */
+REFC_API void* refc_raw(size_t n, const int32_t* st, const int32_t* en, const int32_t* d) {
+    cSuperIntervals* si = createSuperIntervals();
+    for (size_t i = 0; i < n; ++i) addInterval(si, st[i], en[i], d ? d[i] : (int32_t)i);
+    return si;
+}
+REFC_API size_t refc_size(void* p) { return ((cSuperIntervals*)p)->size; }
+/* Arrays in their current order (no branch: set-operation results are unindexed). */
+REFC_API void refc_raw_arrays(void* p, int32_t* st, int32_t* en, int32_t* d) {
+    cSuperIntervals* si = (cSuperIntervals*)p;
+    for (size_t i = 0; i < si->size; ++i) {
+        st[i] = si->starts[i];
+        en[i] = si->ends[i];
+        d[i] = si->data[i];
+    }
+}
+/* intersection with the default combine (keep the first operand's data), difference. */
+REFC_API void* refc_intersection(void* a, void* b) {
+    return intersection((const cSuperIntervals*)a, (cSuperIntervals*)b, NULL);
+}
+REFC_API void* refc_difference(void* a, void* b) {
+    return difference((const cSuperIntervals*)a, (cSuperIntervals*)b);
+}

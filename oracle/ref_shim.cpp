@@ -146,8 +146,31 @@ uint64_t search_batch(const Map<S>* m, int nthreads, size_t nq, const S* qs, con
         return search_batch<S>((Map<S>*)m, t, nq, qs, qe, c);                                  \
     }
 
+// upper_bound_range (superintervals.hpp:524-543): the exit value of `left`, -1 for SIZE_MAX.
+#define REF_EXTRA(S, SFX)                                                                       \
+    REF_API int64_t ref_upper_bound_range_##SFX(void* m, S value, size_t left, size_t right) {  \
+        ((Map<S>*)m)->upper_bound_range(value, left, right);                                   \
+        return left == SIZE_MAX ? -1 : (int64_t)left;                                          \
+    }                                                                                          \
+    REF_API void* ref_raw_##SFX(size_t n, const S* st, const S* en, const int32_t* d) {         \
+        auto* m = new Map<S>();                                                                \
+        for (size_t i = 0; i < n; ++i) m->add(st[i], en[i], d ? d[i] : (int32_t)i);             \
+        return m;                                                                              \
+    }                                                                                          \
+    REF_API void ref_raw_arrays_##SFX(void* m, S* st, S* en, int32_t* d) {                      \
+        arrays<S>((Map<S>*)m, st, en, d, nullptr);                                             \
+    }                                                                                          \
+    REF_API void* ref_intersection_##SFX(void* a, void* b) {                                   \
+        return new Map<S>(((Map<S>*)a)->intersection(*(Map<S>*)b));                            \
+    }                                                                                          \
+    REF_API void* ref_difference_##SFX(void* a, void* b) {                                     \
+        return new Map<S>(((Map<S>*)a)->difference(*(Map<S>*)b));                              \
+    }
+
 REF_INSTANCE(int32_t, i32)
 REF_INSTANCE(int64_t, i64)
+REF_EXTRA(int32_t, i32)
+REF_EXTRA(int64_t, i64)
 
 REF_API void ref_coverage_i32(void* m, int32_t s, int32_t e, uint64_t* c, int32_t* cov) {
     std::pair<size_t, int> r{0, 0};

@@ -257,7 +257,9 @@ class DeviceIndex:
         """Intersection of the intervals (a_s[k], a_e[k]) with this index, superintervals.hpp
         :1158-1174: for each k in order and each hit h in the C++ vector order, the piece
         [max(a_s, starts[h]), min(a_e, ends[h])].  Returns (starts, ends, k, h): the source
-        interval k of the first set and the sorted position h in this index of each piece."""
+        interval k of the first set and the sorted position h in this index of each piece.
+        A piece has start > end when either interval does; callers drop those, as the
+        reference does."""
         import torch
         a_s, a_e, off, idx, t, dev = self._set_op_inputs(a_s, a_e, "vector")
         na, total = a_s.size, int(off[-1])
@@ -631,7 +633,8 @@ class IntervalMap:
             return out
         ps, pe, pk, _ = other._ix.intersection_pieces(s, e)
         for a, b, k in zip(ps.tolist(), pe.tolist(), pk.tolist()):
-            out.add(a, b, vals[k])
+            if a <= b:   # superintervals.hpp:1167: empty pieces (an interval with start > end) are dropped
+                out.add(a, b, vals[k])
         return out
 
     def difference(self, other):
@@ -643,8 +646,9 @@ class IntervalMap:
         out = IntervalMap()
         if s.size == 0:
             return out
-        if other.size() == 0:
-            ps, pe, pk = s, e, np.arange(s.size)
+        if other.size() == 0:   # nothing covered: every interval with start <= end, whole
+            keep = np.flatnonzero(s <= e)
+            ps, pe, pk = s[keep], e[keep], keep
         else:
             ps, pe, pk = other._ix.difference_pieces(s, e)
         for a, b, k in zip(ps.tolist(), pe.tolist(), pk.tolist()):

@@ -18,6 +18,7 @@ SI_ERR_NOT_BUILT = 3
 SI_ERR_CAPACITY = 4
 SI_ERR_NOMEM = 5
 SI_ERR_UNSUPPORTED = 6
+SI_ERR_MISMATCH = 7
 SI_ORDER_DESC = 0
 SI_ORDER_VECTOR = 1
 SI_INDEX_NONE = 0xFFFFFFFF
@@ -84,6 +85,7 @@ _SIGS = {
     "siCountPlanned": (_i, [_vp, _vp, _sz, _vp, _vp]),
     "siSearchOffsetsPlanned": (_i, [_vp, _vp, _sz, _vp, _vp, _vp]),
     "siSearchFillPlanned": (_i, [_vp, _vp, _sz, _i, _vp, _vp, _vp]),
+    "siPlanCheck": (_i, [_vp, _vp, _vp]),
     "siSearchValuesFillPlanned": (_i, [_vp, _vp, _sz, _vp, _vp, _vp]),
     "siSearchFillPlannedBounded": (_i, [_vp, _vp, _sz, _i, _vp, _vp, C.c_uint64, _vp]),
     "siSearchValuesFillPlannedBounded": (_i, [_vp, _vp, _sz, _vp, _vp, C.c_uint64, _vp]),
@@ -126,6 +128,8 @@ _SIGS = {
     "siCountBatch": (_i, [_vp, _vp, _vp, _sz, _vp]),
     "siSearchIdxsBatch": (_i, [_vp, _vp, _vp, _sz, _vp, _vp, _sz, _vp]),
     "siSearchValuesBatch": (_i, [_vp, _vp, _vp, _sz, _vp, _vp, _sz, _vp]),
+    "intersection": (_vp, [_vp, _vp, _vp]),
+    "difference": (_vp, [_vp, _vp]),
 }
 
 for _name, (_res, _args) in _SIGS.items():

@@ -40,7 +40,9 @@ bool ensure_any(R* r, size_t need) {
 }
 
 // The index must exist and be built for queries; the reference leaves querying an
-// unindexed handle undefined, here it reports an error and returns no result.
+// unindexed handle undefined, here it reports an error and returns no result.  addInterval
+// and clearSuperIntervals drop the device index, so a handle changed since its last
+// indexSuperIntervals has none.
 bool ready(const cSuperIntervals* si) {
     if (!si) { si_set_error(SI_ERR_ARG, "null handle"); return false; }
     if (si->size == 0) return false;
@@ -121,7 +123,16 @@ SI_EXPORT void destroySuperIntervals(cSuperIntervals* si) {
     free(si);
 }
 
+// The device index no longer matches the host arrays once they change.
+static void drop_device_index(cSuperIntervals* si) {
+    if (si->dev) {
+        siIndexDestroy(si->dev);
+        si->dev = nullptr;
+    }
+}
+
 SI_EXPORT void clearSuperIntervals(cSuperIntervals* si) {
+    drop_device_index(si);
     si->size = 0;
     si->idx = 0;
     si->startSorted = true;
@@ -138,6 +149,7 @@ SI_EXPORT void reserveSuperIntervals(cSuperIntervals* si, size_t n) {
 }
 
 SI_EXPORT void addInterval(cSuperIntervals* si, int32_t start, int32_t end, int32_t value) {
+    drop_device_index(si);
     if (si->size >= si->capacity) reserveSuperIntervals(si, si->capacity == 0 ? 1 : si->capacity * 2);
     // sortedness flags, c_superintervals.h:408-413
     if (si->startSorted && si->size > 0) {
@@ -304,4 +316,44 @@ SI_EXPORT int siSearchIdxsBatch(cSuperIntervals* si, const int32_t* qs, const in
 SI_EXPORT int siSearchValuesBatch(cSuperIntervals* si, const int32_t* qs, const int32_t* qe, size_t n,
                                   uint64_t* offsets, int32_t* values, size_t cap, uint64_t* total) {
     return batch_search(si, qs, qe, n, offsets, values, true, cap, total);
+}
+
+// ---- Set operations (c_superintervals.h:286, :292) -------------------------------------
+static int32_t keep_first(int32_t a, int32_t) { return a; }
+
+// c_superintervals.h:921-940: for each interval k of si in its current order and each hit h
+// of searchIdxs(other, ...) (descending), the piece [max(starts), min(ends)] when it is not
+// empty, with combine(si->data[k], other->data[h]).  The search and the pieces run on the
+// device; combine is the caller's host function.
+SI_EXPORT cSuperIntervals* intersection(const cSuperIntervals* si, cSuperIntervals* other, cCombineFn combine) {
+    if (!combine) combine = keep_first;
+    cSuperIntervals* out = createSuperIntervals();
+    if (!out || !si || si->size == 0 || !other || other->size == 0 || !ready(other)) return out;
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> pos;
+    std::vector<int32_t> ps, pe;
+    if (si_intersect_host(other->dev, si->starts, si->ends, si->size, off, pos, ps, pe) != SI_OK) return out;
+    for (size_t k = 0; k < si->size; ++k) {
+        for (uint64_t j = off[k]; j < off[k + 1]; ++j) {
+            if (ps[j] <= pe[j]) addInterval(out, ps[j], pe[j], combine(si->data[k], other->data[pos[j]]));
+        }
+    }
+    return out;
+}
+
+// c_superintervals.h:950-975: the parts of each interval of si (current order) that no
+// interval of other covers, with its data.  An empty other covers nothing.
+SI_EXPORT cSuperIntervals* difference(const cSuperIntervals* si, cSuperIntervals* other) {
+    cSuperIntervals* out = createSuperIntervals();
+    if (!out || !si || si->size == 0 || !other) return out;
+    if (other->size == 0) {
+        for (size_t k = 0; k < si->size; ++k)
+            if (si->starts[k] <= si->ends[k]) addInterval(out, si->starts[k], si->ends[k], si->data[k]);
+        return out;
+    }
+    if (!ready(other)) return out;
+    std::vector<int32_t> ps, pe, pd;
+    if (si_difference_host(other->dev, si->starts, si->ends, si->data, si->size, ps, pe, pd) != SI_OK) return out;
+    for (size_t j = 0; j < ps.size(); ++j) addInterval(out, ps[j], pe[j], pd[j]);
+    return out;
 }

@@ -3,7 +3,9 @@
 #pragma once
 
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
+#include <vector>
 
 #include "../../include/si_device.h"
 
@@ -11,3 +13,14 @@
 
 // Records a formatted message for siGetLastError() and returns code.
 int si_set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+// Set operations for the C-ABI on host buffers (int32 indexes), run on the device: A's
+// intervals are searched in ix in descending hit order.
+//   si_intersect_host: off[na + 1] CSR over A's intervals, per hit its position pos and the
+//                      piece [max(starts), min(ends)] (pieces with start > end included).
+//   si_difference_host: the uncovered pieces of every A interval in order, with A's data.
+int si_intersect_host(const siIndex* ix, const int32_t* as, const int32_t* ae, size_t na,
+                      std::vector<uint64_t>& off, std::vector<uint32_t>& pos, std::vector<int32_t>& ps,
+                      std::vector<int32_t>& pe);
+int si_difference_host(const siIndex* ix, const int32_t* as, const int32_t* ae, const int32_t* ad, size_t na,
+                       std::vector<int32_t>& ps, std::vector<int32_t>& pe, std::vector<int32_t>& pd);

@@ -180,6 +180,16 @@ __global__ __launch_bounds__(256) void k_exit(const S* __restrict__ ends, const 
     ex[i] = branch[a];
 }
 
+// Number of intervals with start > end (one atomic per wave that holds any).
+template <typename S>
+__global__ __launch_bounds__(256) void k_count_inverted(const S* __restrict__ st, const S* __restrict__ en, size_t n,
+                                                        unsigned long long* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool inv = i < n && st[i] > en[i];
+    const uint64_t m = __ballot(inv);
+    if (lane_id() == 0 && m) atomicAdd(out, (unsigned long long)__popcll(m));
+}
+
 // branch[i] = nearest j < i with ends[j] >= ends[i], or NONE.
 template <typename S>
 __global__ __launch_bounds__(256) void k_branch(TreeDesc<S> td, uint32_t* __restrict__ branch) {
@@ -387,6 +397,22 @@ int build_impl(siIndex* ix, const void* starts, const void* ends, const int32_t*
     k_exit<S><<<grid_for(npad, 256), 256, 0, st>>>((const S*)ix->d_ends, ix->d_branch, (uint32_t)n, (uint32_t)npad,
                                                  ix->d_exit);
     LAUNCH_CHECK();
+    // intervals with start > end (queries on such an index take the walk, see siIndex)
+    {
+        unsigned long long* d_inv = nullptr;
+        HIP_TRY(hipMalloc(&d_inv, sizeof(unsigned long long)));
+        unsigned long long h_inv = 0;
+        hipError_t e = hipMemsetAsync(d_inv, 0, sizeof(unsigned long long), st);
+        if (e == hipSuccess) {
+            k_count_inverted<S><<<g, 256, 0, st>>>((const S*)ix->d_starts, (const S*)ix->d_ends, n, d_inv);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(&h_inv, d_inv, sizeof(h_inv), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        hipFree(d_inv);
+        if (e != hipSuccess) return si_set_error(SI_ERR_HIP, "inverted count: %s", hipGetErrorString(e));
+        ix->n_inverted = (size_t)h_inv;
+    }
     // upper_bound search tree over the starts
     {
         const int rc = build_key_tree<S>((const S*)ix->d_starts, n, ix->ub, st);

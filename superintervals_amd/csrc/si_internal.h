@@ -45,6 +45,10 @@ struct siIndex {
     void* d_ends_sorted = nullptr;   // ends in ascending order (rank count)
     uint32_t* d_ends32 = nullptr;    // int64 only, ends - ends32_base when they span < 2^32
     int64_t ends32_base = 0;
+    // intervals with start > end: the rank form of count and the guaranteed-hit run
+    // [first start >= s, upper_bound] both assume start <= end, so queries on an index
+    // holding any such interval take the walk
+    size_t n_inverted = 0;
     KeyTree ub;   // search tree over the starts (upper_bound)
     KeyTree eb;   // search tree over the sorted ends (rank count)
     int nlev = 0;
@@ -58,6 +62,7 @@ struct siIndex {
         d_starts = d_ends = d_tree = d_ends_sorted = nullptr;
         d_ends32 = nullptr;
         ends32_base = 0;
+        n_inverted = 0;
         ub.release();
         eb.release();
         d_branch = d_exit = d_perm = nullptr;

@@ -54,7 +54,8 @@ struct HeavyEntry;
 
 template <typename S>
 struct PlanView {
-    uint32_t* counter;   // [0] tiles taken, [1] heavy entries written, [2] heavy entries taken
+    uint32_t* counter;   // [0] tiles taken, [1] heavy entries written, [2] heavy entries taken,
+                         // [3] set by a fill pass when a query's hits differ from its offsets
     uint32_t* q;         // input query number of sorted position k
     S* s;                // query start of sorted position k
     S* e;                // query end of sorted position k
@@ -66,14 +67,17 @@ struct PlanView {
 //   [256 B counters][q: 4 nq][s: cb nq][e: cb nq][idx: 4 nq][scratch: keys 2 x 8 nq,
 //   vals 2 x 4 nq, radix temporaries]; the scratch is reused as the offsets scan workspace.
 // A query whose walk exceeds the per-lane step budget, parked for the wave-cooperative
-// kernel: sorted position, current position, leading-run top and phase, and the count so
-// far (count mode) or the next output offset (fill modes).
+// kernel: sorted position, current position, leading-run top and phase, the count so far
+// (count mode) or the next output offset (fill modes), and in fill modes the hits still
+// expected by the query's offsets.
 struct HeavyEntry {
     uint32_t k;
     uint32_t cur;
     uint32_t top;
     uint32_t lead;
     uint64_t acc;
+    uint32_t left;
+    uint32_t pad;
 };
 
 struct PlanLayout {
@@ -554,7 +558,14 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
                     kq = k;
                     steps = 0;
                     has = true;
-                    if (MODE != W_COUNT) { o = min(offsets[q], cap); oend = min(offsets[q + 1], cap); }
+                    if (MODE != W_COUNT) {
+                        // cnt counts down the hits the offsets promise; the output cursor and
+                        // segment end are cut at the capacity
+                        const uint64_t o0 = offsets[q], o1 = offsets[q + 1];
+                        cnt = (uint32_t)(o1 - o0);
+                        o = min(o0, cap);
+                        oend = min(o1, cap);
+                    }
                     if (MODE == W_VECTOR) { top = cur; lead = true; }
                 }
                 const uint32_t take = min((uint32_t)__popcll(nm), pool_left);
@@ -592,6 +603,7 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
                         lo = cur + 1u;
                     }
                     if (inv != 0 || cur == NONE) {
+                        cnt -= top - lo + 1u;
                         c_emit = (uint32_t)min((uint64_t)(top - lo + 1u), oend - o);   // never past the segment
                         e_base = lo;
                         e_asc = true;
@@ -603,6 +615,7 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
                     if (MODE == W_COUNT) {
                         cnt += (uint32_t)__popc(m);
                     } else {
+                        cnt -= (uint32_t)__popc(m);
                         c_emit = (uint32_t)min((uint64_t)__popc(m), oend - o);   // never past the segment
                         e_m = m;
                         e_base = b0;
@@ -648,6 +661,8 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
                 if (MODE == W_COUNT) {
                     if (counts) counts[q] = cnt;
                     if (counts64) counts64[q] = cnt;
+                } else if (cnt != 0u) {
+                    atomicOr(&pv.counter[3], 1u);   // the walk's hits differ from the offsets
                 }
                 lead = false;
                 has = false;
@@ -660,6 +675,8 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
                 he.top = top;
                 he.lead = lead ? 1u : 0u;
                 he.acc = (MODE == W_COUNT) ? (uint64_t)cnt : o;
+                he.left = cnt;
+                he.pad = 0u;
                 pv.heavy[h] = he;
                 lead = false;
                 has = false;
@@ -720,6 +737,7 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanV
         uint32_t cur = __builtin_amdgcn_readfirstlane(he.cur);
         const uint32_t top = __builtin_amdgcn_readfirstlane(he.top);
         bool lead = __builtin_amdgcn_readfirstlane(he.lead) != 0;
+        uint32_t left = __builtin_amdgcn_readfirstlane(he.left);   // fill modes: hits still expected
         uint64_t acc = he.acc;
         const uint64_t oend = (MODE != W_COUNT) ? min(offsets[q + 1], cap) : 0;
         while (cur != NONE) {
@@ -762,6 +780,7 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanV
                         lo = cur + 1u;
                     }
                     if (inv != 0 || cur == NONE) {
+                        left -= top - lo + 1u;
                         const uint64_t c = min((uint64_t)(top - lo + 1u), oend - acc);
                         for (uint64_t x = lane; x < c; x += WAVE) out_idx[acc + x] = lo + (uint32_t)x;
                         acc += c;
@@ -774,6 +793,7 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanV
                 if (MODE == W_COUNT) {
                     acc += nh;
                 } else {
+                    left -= nh;
                     if (hit) {
                         const uint64_t r = (uint64_t)__popcll(m & ~le_mask);   // hits above this lane
                         if (acc + r < oend) {
@@ -798,6 +818,8 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanV
             // every lane stores the same value (no lane-dependent branch in the loop)
             if (counts) counts[q] = (uint32_t)acc;
             if (counts64) counts64[q] = acc;
+        } else if (left != 0u) {
+            atomicOr(&pv.counter[3], 1u);   // wave-uniform test: the walk's hits differ from the offsets
         }
     }
 }
@@ -1085,7 +1107,7 @@ int launch_walk(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint
     uint32_t tile = 0, ntiles = 0;
     unsigned grid = 0;
     walk_shape<S, MODE>(nq, &tile, &ntiles, &grid);
-    k_zero_u32<<<1, 64, 0, st>>>(pv.counter, 3);
+    k_zero_u32<<<1, 64, 0, st>>>(pv.counter, 4);
     LAUNCH_CHECK();
     k_walk<S, MODE><<<grid, T_THREADS, 0, st>>>(
         ix->view<S>(), pv, (uint32_t)nq, tile, ntiles, g_tune.lane_steps, pv.counter, counts, counts64,
@@ -1106,6 +1128,10 @@ int launch_walk(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint
     LAUNCH_CHECK();
     return SI_OK;
 }
+
+// The rank form counts (upper_bound + 1) - #{ends < s}, which assumes that every interval
+// ending before s starts before e: true unless the index holds an interval with start > end.
+inline bool rank_exact(const siIndex* ix) { return ix->n_inverted == 0; }
 
 int check_plan(const siIndex* ix, const void* plan, size_t plan_bytes, size_t nq) {
     if (!plan) return si_set_error(SI_ERR_ARG, "null plan");
@@ -1209,7 +1235,7 @@ SI_EXPORT int siCountPlanned(const siIndex* ix, void* plan, size_t nq, uint32_t*
     if (!plan) return si_set_error(SI_ERR_ARG, "null plan");
     if (nq && !counts) return si_set_error(SI_ERR_ARG, "null counts");
     hipStream_t st = (hipStream_t)stream;
-    if (g_tune.count_rank)
+    if (g_tune.count_rank && rank_exact(ix))
         return DISPATCH_S(ix, (launch_rank<S>(ix, plan, nq, counts, nullptr, st)));
     return DISPATCH_S(ix, (launch_walk<S, W_COUNT>(ix, plan, nq, counts, nullptr, nullptr, nullptr, nullptr, st)));
 }
@@ -1222,7 +1248,7 @@ SI_EXPORT int siSearchOffsetsPlanned(const siIndex* ix, void* plan, size_t nq, u
     hipStream_t st = (hipStream_t)stream;
     const PlanLayout L(ix->cb, nq);
     uint64_t* ws = (uint64_t*)((char*)plan + L.off_scratch);
-    if (g_tune.offsets_rank) {
+    if (g_tune.offsets_rank && rank_exact(ix)) {
         // 32-bit counts (the caller's, or the heavy-list area of the plan, which the rank
         // form does not use), then one scan to 64-bit offsets: half the scattered bytes
         uint32_t* c32 = counts ? counts : (uint32_t*)((char*)plan + L.off_heavy);
@@ -1248,6 +1274,18 @@ SI_EXPORT int siSearchFillPlannedBounded(const siIndex* ix, void* plan, size_t n
                                                       capacity)));
     return DISPATCH_S(ix, (launch_walk<S, W_VECTOR>(ix, plan, nq, nullptr, nullptr, offsets, idx, nullptr, st,
                                                     capacity)));
+}
+
+SI_EXPORT int siPlanCheck(const siIndex* ix, const void* plan, void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (!plan) return si_set_error(SI_ERR_ARG, "null plan");
+    hipStream_t st = (hipStream_t)stream;
+    uint32_t flag = 0;
+    HIP_TRY(hipMemcpyAsync(&flag, (const uint32_t*)plan + 3, sizeof(flag), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (flag) return si_set_error(SI_ERR_MISMATCH, "a fill pass found a different number of hits than its offsets give");
+    return SI_OK;
 }
 
 SI_EXPORT int siSearchFillPlanned(const siIndex* ix, void* plan, size_t nq, int order,
@@ -1421,11 +1459,12 @@ __global__ __launch_bounds__(Q_THREADS) void k_intersect_pieces(IndexView<S> ix,
 
 // Difference pieces (superintervals.hpp:1188-1211): the uncovered parts of A's interval k.
 // The reference sorts B's hits ascending by (start, end); a descending search lists them
-// in exactly the reverse of that order (positions are sorted by start asc, end desc), so
-// the sweep reads the hits backwards.  Once a hit covers A's interval to its end nothing
-// more is uncovered (every later piece would start past the end), so the sweep stops
-// there instead of stepping the cursor to end + 1, which would wrap at the type's
-// maximum.  FILL = false counts the pieces, true writes them.
+// in the reverse of position order, i.e. ascending by (start, end desc), and the sweep's
+// pieces do not depend on the order of hits with equal starts (the first of them emits any
+// gap, the cursor ends at the largest end + 1 either way), so the sweep reads the hits
+// backwards.  The cursor is kept in 64 bits, so end + 1 never wraps for int32; for int64 a
+// hit ending at the type's maximum covers the rest of the interval and ends the sweep.
+// FILL = false counts the pieces, true writes them.
 template <typename S, bool FILL>
 __global__ __launch_bounds__(Q_THREADS) void k_difference_pieces(IndexView<S> ix, const S* __restrict__ as,
                                                                  const S* __restrict__ ae,
@@ -1437,11 +1476,12 @@ __global__ __launch_bounds__(Q_THREADS) void k_difference_pieces(IndexView<S> ix
                                                                  int32_t* __restrict__ od) {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= na) return;
+    constexpr int64_t SMAX = (sizeof(S) == 4) ? (int64_t)INT32_MAX : INT64_MAX;
     const int64_t s = (int64_t)as[k], e = (int64_t)ae[k];
     const int32_t d = ad ? ad[k] : 0;
     uint64_t w = FILL ? pieces[k] : 0;
     int64_t cursor = s;
-    bool past = false;   // cursor is beyond e
+    bool past = false;   // cursor is beyond the type's maximum
     for (uint64_t j = off[k + 1]; j-- > off[k];) {
         const uint32_t h = idx[j];
         const int64_t cs = max((int64_t)ix.starts[h], s), ce = min((int64_t)ix.ends[h], e);
@@ -1453,7 +1493,7 @@ __global__ __launch_bounds__(Q_THREADS) void k_difference_pieces(IndexView<S> ix
             }
             ++w;
         }
-        if (ce >= e) {
+        if (sizeof(S) == 8 && ce == SMAX) {
             past = true;
             break;
         }
@@ -1666,6 +1706,87 @@ SI_EXPORT int siSearchHost(const siIndex* ix, const void* qs, const void* qe, si
     if (values) rc = siSearchValuesFillPlanned(ix, dplan.p, nq, (const uint64_t*)doff.p, (int32_t*)dout.p, nullptr);
     else rc = siSearchFillPlanned(ix, dplan.p, nq, order, (const uint64_t*)doff.p, (uint32_t*)dout.p, nullptr);
     if (rc) return rc;
+    if ((rc = siPlanCheck(ix, dplan.p, nullptr))) return rc;
     HIP_TRY(hipMemcpy(values ? (void*)values : (void*)idx, dout.p, tot * 4, hipMemcpyDeviceToHost));
+    return SI_OK;
+}
+
+// ---- Set operations on host buffers (the C-ABI's intersection / difference) ----------
+namespace {
+// Uploads A's int32 intervals as queries and runs plan -> offsets -> descending fill on
+// the device; off receives the CSR offsets (host), didx the positions (device).
+int search_desc_host(const siIndex* ix, const int32_t* as, const int32_t* ae, size_t na, DevBuf& dq,
+                     DevBuf& dplan, DevBuf& doff, DevBuf& didx, std::vector<uint64_t>& off) {
+    const size_t pb = siPlanBytes(ix, na);
+    int rc;
+    if ((rc = upload_queries(ix, as, ae, na, dq)) || (rc = doff.alloc((na + 1) * 8)) || (rc = dplan.alloc(pb)))
+        return rc;
+    if ((rc = siPlanQueries(ix, dq.p, (char*)dq.p + na * 4, na, dplan.p, pb, nullptr))) return rc;
+    if ((rc = siSearchOffsetsPlanned(ix, dplan.p, na, nullptr, (uint64_t*)doff.p, nullptr))) return rc;
+    off.resize(na + 1);
+    HIP_TRY(hipMemcpy(off.data(), doff.p, (na + 1) * 8, hipMemcpyDeviceToHost));
+    if ((rc = didx.alloc(off[na] * 4))) return rc;
+    if (off[na] == 0) return SI_OK;
+    if ((rc = siSearchFillPlanned(ix, dplan.p, na, SI_ORDER_DESC, (const uint64_t*)doff.p, (uint32_t*)didx.p,
+                                  nullptr)))
+        return rc;
+    return siPlanCheck(ix, dplan.p, nullptr);
+}
+}  // namespace
+
+int si_intersect_host(const siIndex* ix, const int32_t* as, const int32_t* ae, size_t na,
+                      std::vector<uint64_t>& off, std::vector<uint32_t>& pos, std::vector<int32_t>& ps,
+                      std::vector<int32_t>& pe) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (ix->cb != 4) return si_set_error(SI_ERR_UNSUPPORTED, "the C-ABI set operations take int32 indexes");
+    off.assign(na + 1, 0);
+    pos.clear(); ps.clear(); pe.clear();
+    if (na == 0) return SI_OK;
+    DevBuf dq, dplan, doff, didx, dps, dpe;
+    if ((rc = search_desc_host(ix, as, ae, na, dq, dplan, doff, didx, off))) return rc;
+    const uint64_t total = off[na];
+    if (total == 0) return SI_OK;
+    if ((rc = dps.alloc(total * 4)) || (rc = dpe.alloc(total * 4))) return rc;
+    if ((rc = siIntersectPieces(ix, dq.p, (char*)dq.p + na * 4, nullptr, na, (const uint64_t*)doff.p,
+                                (const uint32_t*)didx.p, dps.p, dpe.p, nullptr, nullptr, nullptr)))
+        return rc;
+    pos.resize(total); ps.resize(total); pe.resize(total);
+    HIP_TRY(hipMemcpy(pos.data(), didx.p, total * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ps.data(), dps.p, total * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(pe.data(), dpe.p, total * 4, hipMemcpyDeviceToHost));
+    return SI_OK;
+}
+
+int si_difference_host(const siIndex* ix, const int32_t* as, const int32_t* ae, const int32_t* ad, size_t na,
+                       std::vector<int32_t>& ps, std::vector<int32_t>& pe, std::vector<int32_t>& pd) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (ix->cb != 4) return si_set_error(SI_ERR_UNSUPPORTED, "the C-ABI set operations take int32 indexes");
+    ps.clear(); pe.clear(); pd.clear();
+    if (na == 0) return SI_OK;
+    std::vector<uint64_t> off;
+    DevBuf dq, dplan, doff, didx, dpoff, dws, dad, dps, dpe, dpd;
+    if ((rc = search_desc_host(ix, as, ae, na, dq, dplan, doff, didx, off))) return rc;
+    const size_t wsb = siSearchWorkspaceBytes(na);
+    if ((rc = dpoff.alloc((na + 1) * 8)) || (rc = dws.alloc(wsb)) || (rc = dad.alloc(na * 4))) return rc;
+    HIP_TRY(hipMemcpy(dad.p, ad, na * 4, hipMemcpyHostToDevice));
+    const void* dqs = dq.p;
+    const void* dqe = (const char*)dq.p + na * 4;
+    if ((rc = siDifferenceOffsets(ix, dqs, dqe, na, (const uint64_t*)doff.p, (const uint32_t*)didx.p,
+                                  (uint64_t*)dpoff.p, dws.p, wsb, nullptr)))
+        return rc;
+    uint64_t np_ = 0;
+    HIP_TRY(hipMemcpy(&np_, (const uint64_t*)dpoff.p + na, 8, hipMemcpyDeviceToHost));
+    if (np_ == 0) return SI_OK;
+    if ((rc = dps.alloc(np_ * 4)) || (rc = dpe.alloc(np_ * 4)) || (rc = dpd.alloc(np_ * 4))) return rc;
+    if ((rc = siDifferenceFill(ix, dqs, dqe, (const int32_t*)dad.p, na, (const uint64_t*)doff.p,
+                               (const uint32_t*)didx.p, (const uint64_t*)dpoff.p, dps.p, dpe.p, (int32_t*)dpd.p,
+                               nullptr)))
+        return rc;
+    ps.resize(np_); pe.resize(np_); pd.resize(np_);
+    HIP_TRY(hipMemcpy(ps.data(), dps.p, np_ * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(pe.data(), dpe.p, np_ * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(pd.data(), dpd.p, np_ * 4, hipMemcpyDeviceToHost));
     return SI_OK;
 }

@@ -51,3 +51,36 @@ def sha(*arrays):
 def orc():
     import oracle
     return oracle.Oracle()
+
+
+def sweep_difference(a_s, a_e, ob_s, ob_e, hits):
+    """The reference's difference sweep (superintervals.hpp:1188-1211, c_superintervals.h:
+    950-975) in Python integers: for each interval k of A, B's hit keys sorted ascending by
+    (start, end), pieces (start, end, k) of what they leave uncovered."""
+    out = []
+    for k in range(len(a_s)):
+        cover = sorted((int(ob_s[h]), int(ob_e[h])) for h in hits[k])
+        s, e = int(a_s[k]), int(a_e[k])
+        cursor = s
+        for cs0, ce0 in cover:
+            cs, ce = max(cs0, s), min(ce0, e)
+            if cs > cursor:
+                out.append((cursor, cs - 1, k))
+            if ce + 1 > cursor:
+                cursor = ce + 1
+        if cursor <= e:
+            out.append((cursor, e, k))
+    return out
+
+
+def overlap_pieces(a_s, a_e, ob_s, ob_e, hits):
+    """The reference's intersection (superintervals.hpp:1158-1174, c_superintervals.h:
+    921-940): for each interval k of A and each hit h in the given order, the piece
+    (max(starts), min(ends), k, h) when it is not empty."""
+    out = []
+    for k in range(len(a_s)):
+        for h in hits[k]:
+            s, e = max(int(a_s[k]), int(ob_s[h])), min(int(a_e[k]), int(ob_e[h]))
+            if s <= e:
+                out.append((s, e, k, int(h)))
+    return out

@@ -223,7 +223,78 @@ void set_ops() {
     CHECK(cd.size() == 1 && cd.starts[0] == big - 5 && cd.ends[0] == big - 3 && cd.data[0] == 7);
 }
 
-int main() {
+// ---- file-driven checks (vectors written by the Python tests; int64 words) ---------------
+static std::vector<int64_t> read_words(const char* path) {
+    std::vector<int64_t> w;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return w;
+    int64_t x;
+    while (std::fread(&x, sizeof(x), 1, f) == 1) w.push_back(x);
+    std::fclose(f);
+    return w;
+}
+
+static bool write_words(const char* path, const std::vector<int64_t>& w) {
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return false;
+    const bool ok = std::fwrite(w.data(), sizeof(int64_t), w.size(), f) == w.size();
+    return std::fclose(f) == 0 && ok;
+}
+
+// upper_bound_range on a map whose starts are given (host code only, no GPU):
+// in = [n, starts[n], k, value[k], left[k], right[k]], out = exit left[k] (-1 = SIZE_MAX).
+template <typename S>
+static int ubr_mode(const std::vector<int64_t>& w, std::vector<int64_t>& out) {
+    size_t p = 1;
+    const size_t n = (size_t)w[p++];
+    si::IntervalMap<S, int> m;
+    for (size_t i = 0; i < n; ++i) m.starts.push_back((S)w[p++]);
+    const size_t k = (size_t)w[p++];
+    const int64_t* value = &w[p];
+    const int64_t* left = value + k;
+    const int64_t* right = left + k;
+    for (size_t q = 0; q < k; ++q) {
+        size_t l = (size_t)left[q];
+        m.upper_bound_range((S)value[q], l, (size_t)right[q]);
+        out.push_back(l == SIZE_MAX ? -1 : (int64_t)l);
+    }
+    return 0;
+}
+
+// intersection and difference of A (in order) with B (built), on the GPU:
+// in = [na, a_s, a_e, a_d, nb, b_s, b_e, b_d], out = [ni, s, e, d, nd, s, e, d].
+template <typename S>
+static int setops_mode(const std::vector<int64_t>& w, std::vector<int64_t>& out) {
+    size_t p = 1;
+    si::IntervalMap<S, int> a, b;
+    const size_t na = (size_t)w[p++];
+    for (size_t i = 0; i < na; ++i) a.add((S)w[p + i], (S)w[p + na + i], (int)w[p + 2 * na + i]);
+    p += 3 * na;
+    const size_t nb = (size_t)w[p++];
+    for (size_t i = 0; i < nb; ++i) b.add((S)w[p + i], (S)w[p + nb + i], (int)w[p + 2 * nb + i]);
+    b.build();
+    for (auto* r : {new si::IntervalMap<S, int>(a.intersection(b)), new si::IntervalMap<S, int>(a.difference(b))}) {
+        out.push_back((int64_t)r->starts.size());
+        for (S x : r->starts) out.push_back((int64_t)x);
+        for (S x : r->ends) out.push_back((int64_t)x);
+        for (int x : r->data) out.push_back((int64_t)x);
+        delete r;
+    }
+    return 0;
+}
+
+int main(int argc, char** argv) {
+    if (argc == 4) {   // <ubr|setops> <in> <out>; word 0 of in = coordinate bytes
+        const std::string mode = argv[1];
+        const std::vector<int64_t> w = read_words(argv[2]);
+        if (w.empty()) return 3;
+        std::vector<int64_t> out;
+        int rc;
+        if (mode == "ubr") rc = (w[0] == 4) ? ubr_mode<int32_t>(w, out) : ubr_mode<int64_t>(w, out);
+        else if (mode == "setops") rc = (w[0] == 4) ? setops_mode<int32_t>(w, out) : setops_mode<int64_t>(w, out);
+        else return 3;
+        return (rc == 0 && write_words(argv[3], out)) ? 0 : 1;
+    }
     siIndex* probe = siIndexCreate(4);
     if (!probe) {
         std::fprintf(stderr, "no GPU: %s\n", siGetLastError());

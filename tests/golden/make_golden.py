@@ -7,7 +7,8 @@ The fixtures hold inputs (or the SHA-256 of generated inputs) and the reference'
 outputs: sorted arrays, branch, counts, upper_bound, has_overlaps, coverage and
 order-sensitive per-query hashes of search_idxs (both orders) and searchValues.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py                 (every fixture)
+    python tests/golden/make_golden.py --extras-only   (extras.npz only)
 """
 from __future__ import annotations
 
@@ -66,11 +67,88 @@ def cases():
     yield "c2_1m", dict(config=2, n=1_000_000, nq=100_000, seed=8, qseed=9, sorted=1, transform="none"), s, e, qs, qe, False
 
 
+def csr(lists, dtype=np.int64):
+    off = np.concatenate([[0], np.cumsum([len(x) for x in lists])]).astype(np.uint64)
+    flat = np.concatenate(lists).astype(dtype) if any(len(x) for x in lists) else np.zeros(0, dtype)
+    return off, flat
+
+
+def extras(R):
+    """tests/golden/extras.npz: reference outputs for upper_bound_range, an index holding
+    intervals with start > end, and the set operations (C and C++).  Inputs are stored."""
+    rec = {}
+    rng = np.random.default_rng(2024)
+    # upper_bound_range (superintervals.hpp:524-543) on built maps, int32 and int64
+    for sfx, dt, off in (("i32", np.int32, 0), ("i64", np.int64, np.int64(1) << 33)):
+        n = 3000
+        s = np.sort(rng.integers(0, 200_000, n)).astype(dt) + dt(off)
+        e = (s + rng.integers(0, 500, n)).astype(dt)
+        m = oracle.RefMap(R, s, e)
+        a = m.arrays()
+        k = 6_000
+        right = rng.integers(0, n, k)
+        left = np.maximum(right - rng.integers(0, 3, k) * rng.integers(0, 60, k), 0)
+        pick = rng.integers(0, n, k)
+        value = np.where(rng.random(k) < 0.5, a["starts"][pick],
+                         rng.integers(-1000, 201_000, k).astype(dt) + dt(off)).astype(dt)
+        out = np.array([m.upper_bound_range(v, l_, r_) for v, l_, r_ in zip(value, left, right)], np.int64)
+        rec[f"ubr_{sfx}_starts"] = a["starts"]
+        rec[f"ubr_{sfx}_value"], rec[f"ubr_{sfx}_left"], rec[f"ubr_{sfx}_right"] = value, left.astype(np.int64), right.astype(np.int64)
+        rec[f"ubr_{sfx}_out"] = out
+    # an index in which 15 % of the intervals have start > end (unsorted input), queries with
+    # 5 % inverted: C++ counts, both C++ orders, C searchIdxs and searchValues
+    n, nq = 3000, 2000
+    s = rng.integers(0, 50_000, n).astype(np.int32)
+    ln = rng.integers(0, 300, n)
+    inv = rng.random(n) < 0.15
+    e = np.where(inv, s - 1 - rng.integers(0, 300, n), s + ln).astype(np.int32)
+    qs = rng.integers(-100, 50_500, nq).astype(np.int32)
+    qe = np.where(rng.random(nq) < 0.05, qs - rng.integers(1, 200, nq), qs + rng.integers(0, 1500, nq)).astype(np.int32)
+    data = np.arange(n, dtype=np.int32)
+    m = oracle.RefMap(R, s, e, data)
+    c = oracle.RefCMap(R, s, e, data)
+    rec["inv_starts"], rec["inv_ends"], rec["inv_qs"], rec["inv_qe"] = s, e, qs, qe
+    rec["inv_counts"] = np.array([m.count(x, y) for x, y in zip(qs, qe)], np.int64)
+    rec["inv_desc_off"], rec["inv_desc_idx"] = csr([m.search(x, y, 0) for x, y in zip(qs, qe)], np.int32)
+    rec["inv_vec_off"], rec["inv_vec_idx"] = csr([m.search(x, y, 1) for x, y in zip(qs, qe)], np.int32)
+    rec["inv_cidx_off"], rec["inv_cidx"] = csr([c.search(x, y, 0) for x, y in zip(qs, qe)], np.int32)
+    rec["inv_cval_off"], rec["inv_cval"] = csr([c.search(x, y, 1) for x, y in zip(qs, qe)], np.int32)
+    # set operations: A in insertion order (10 % inverted), B indexed (5 % inverted)
+    for sfx, dt in (("i32", np.int32), ("i64", np.int64)):
+        na, nb = 700, 900
+        base = dt(0) if dt == np.int32 else dt(3) << 35
+        a_s = (rng.integers(0, 40_000, na) + base).astype(dt)
+        a_e = np.where(rng.random(na) < 0.10, a_s - rng.integers(1, 50, na), a_s + rng.integers(0, 400, na)).astype(dt)
+        b_s = (rng.integers(0, 40_000, nb) + base).astype(dt)
+        b_e = np.where(rng.random(nb) < 0.05, b_s - rng.integers(1, 50, nb), b_s + rng.integers(0, 80, nb)).astype(dt)
+        a_d = (1000 + np.arange(na)).astype(np.int32)
+        b_d = (5000 + np.arange(nb)).astype(np.int32)
+        rec[f"set_{sfx}_a"] = np.stack([a_s, a_e]).astype(dt)
+        rec[f"set_{sfx}_b"] = np.stack([b_s, b_e]).astype(dt)
+        rec[f"set_{sfx}_ad"], rec[f"set_{sfx}_bd"] = a_d, b_d
+        ma = oracle.RefMap.raw(R, a_s, a_e, a_d)
+        mb = oracle.RefMap(R, b_s, b_e, b_d)
+        for op in ("intersection", "difference"):
+            os_, oe, od = ma.set_op(mb, op)
+            rec[f"set_{sfx}_cpp_{op}"] = np.stack([os_.astype(np.int64), oe.astype(np.int64), od.astype(np.int64)])
+        if dt == np.int32:
+            ca = oracle.RefCMap.raw(R, a_s, a_e, a_d)
+            cb = oracle.RefCMap(R, b_s, b_e, b_d)
+            for op in ("intersection", "difference"):
+                os_, oe, od = ca.set_op(cb, op)
+                rec[f"set_{sfx}_c_{op}"] = np.stack([os_.astype(np.int64), oe.astype(np.int64), od.astype(np.int64)])
+    np.savez_compressed(os.path.join(OUT, "extras.npz"), **rec)
+
+
 def main():
     if not oracle.Reference.available():
         raise SystemExit("oracle/_ref/libsiref.so is missing: run make -f oracle/Makefile ref")
     R = oracle.Reference()
     O = oracle.Oracle()
+    extras(R)
+    if "--extras-only" in sys.argv:
+        print("done")
+        return
     for name, meta, s, e, qs, qe, store in cases():
         print("case", name, flush=True)
         data = np.arange(s.size, dtype=np.int32)

@@ -11,6 +11,7 @@ def declared_functions():
     for h in ("c_superintervals.h", "si_device.h"):
         txt = open(os.path.join(ROOT, "include", h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        txt = re.sub(r"^typedef[^;]*;", "", txt, flags=re.M)   # function-pointer types
         for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(\w+)\s*\(", txt, flags=re.M):
             name = m.group(1)
             if name not in ("if", "while", "for", "return", "sizeof", "defined"):

@@ -617,6 +617,129 @@ def test_capi_empty_and_unindexed(sia):
     L.destroySuperIntervals(h)
 
 
+def test_capi_stale_handle_after_clear_and_add(sia):
+    """A handle changed after indexSuperIntervals has no device index until it is indexed
+    again, even when the new contents have the old size."""
+    N = _capi()
+    L = N.lib
+    h = L.createSuperIntervals()
+    for s0, e0 in [(1, 10), (5, 20), (30, 40)]:
+        L.addInterval(h, s0, e0, 0)
+    L.indexSuperIntervals(h)
+    assert L.countOverlaps(h, 8, 9) == 2
+    L.clearSuperIntervals(h)
+    for s0, e0 in [(100, 110), (105, 120), (130, 140)]:
+        L.addInterval(h, s0, e0, 0)
+    assert L.countOverlaps(h, 8, 9) == 0 and L.countOverlaps(h, 106, 107) == 0
+    assert b"indexSuperIntervals" in L.siGetLastError()
+    L.indexSuperIntervals(h)
+    assert L.countOverlaps(h, 106, 107) == 2 and L.countOverlaps(h, 8, 9) == 0
+    L.addInterval(h, 1, 500, 0)           # one more after indexing: stale again
+    assert L.countOverlaps(h, 106, 107) == 0
+    L.indexSuperIntervals(h)
+    assert L.countOverlaps(h, 106, 107) == 3
+    L.destroySuperIntervals(h)
+
+
+def _extras():
+    import os
+    from conftest import GOLDEN
+    return dict(np.load(os.path.join(GOLDEN, "extras.npz")))
+
+
+@pytest.mark.parametrize("lane_steps", [256, 2])
+def test_index_with_inverted_intervals_golden(sia, orc, lane_steps):
+    """An index in which 15 % of the intervals have start > end: every count form and both
+    orders against the reference's outputs (the rank forms are not exact there, so the
+    library walks)."""
+    from superintervals_amd import autotune
+    print(f"[gpu] inverted intervals lane_steps={lane_steps}", flush=True)
+    x = _extras()
+    s, e, qs, qe = x["inv_starts"], x["inv_ends"], x["inv_qs"], x["inv_qe"]
+    ix = _dev_index(sia, s, e, np.arange(s.size, dtype=np.int32))
+    saved = autotune.current()
+    try:
+        autotune.apply({"lane_steps": lane_steps})
+        for cr in (0, 1):
+            for orank in (0, 1):
+                autotune.apply({"count_rank": cr, "offsets_rank": orank})
+                assert np.array_equal(ix.count(qs, qe).astype(np.int64), x["inv_counts"]), (cr, orank)
+                off, idx = ix.search(qs, qe, order="desc")
+                assert np.array_equal(off, x["inv_desc_off"]) and np.array_equal(idx.astype(np.int32), x["inv_desc_idx"])
+                off, idx = ix.search(qs, qe, order="vector")
+                assert np.array_equal(off, x["inv_vec_off"]) and np.array_equal(idx.astype(np.int32), x["inv_vec_idx"])
+                off, vals = ix.search(qs, qe, values=True)
+                assert np.array_equal(off, x["inv_cval_off"]) and np.array_equal(vals, x["inv_cval"])
+    finally:
+        autotune.apply(saved)
+    ix.close()
+    # the C-ABI on the same intervals
+    N = _capi()
+    L = N.lib
+    h = L.createSuperIntervals()
+    for i in range(s.size):
+        L.addInterval(h, int(s[i]), int(e[i]), i)
+    L.indexSuperIntervals(h)
+    res = N.cIndexResult()
+    off = x["inv_cidx_off"].astype(np.int64)
+    for q in range(0, qs.size, 7):
+        L.clearIndexResult(C.byref(res))
+        L.searchIdxs(h, int(qs[q]), int(qe[q]), C.byref(res))
+        assert [res.data[k] for k in range(res.size)] == x["inv_cidx"][off[q]:off[q + 1]].tolist()
+        assert L.countOverlaps(h, int(qs[q]), int(qe[q])) == x["inv_counts"][q]
+    L.destroyIndexResult(C.byref(res))
+    L.destroySuperIntervals(h)
+
+
+@pytest.mark.parametrize("order", ["desc", "vector"])
+def test_fill_reports_offsets_that_disagree(sia, orc, order):
+    """A fill never writes outside a query's segment; with offsets that disagree with the
+    index (one query given a slot too many, or one too few) siPlanCheck reports it."""
+    import torch
+    N = _capi()
+    print(f"[gpu] fill/offsets mismatch order={order}", flush=True)
+    rng = np.random.default_rng(5)
+    s = np.sort(rng.integers(0, 100_000, 20_000)).astype(np.int32)
+    e = (s + rng.integers(0, 500, s.size)).astype(np.int32)
+    qs = rng.integers(0, 100_000, 5000).astype(np.int32)
+    qe = (qs + rng.integers(0, 300, qs.size)).astype(np.int32)
+    ix = _dev_index(sia, s, e)
+    dev = torch.device("cuda:0")
+    tqs, tqe = torch.from_numpy(qs).to(dev), torch.from_numpy(qe).to(dev)
+    nq = qs.size
+    plan = torch.empty(ix.plan_bytes(nq), dtype=torch.uint8, device=dev)
+    off = torch.empty(nq + 1, dtype=torch.int64, device=dev)
+    ix.plan_device(tqs, tqe, plan)
+    ix.offsets_planned(plan, nq, off)
+    torch.cuda.synchronize()
+    counts = np.diff(off.cpu().numpy())
+    q = int(np.flatnonzero(counts >= 2)[0])
+    total = int(off[-1].item())
+    idx = torch.full((total + 1,), -1, dtype=torch.int32, device=dev)
+    ix.fill_planned(plan, nq, off, idx, order=order)
+    assert N.lib.siPlanCheck(ix.handle, plan.data_ptr(), None) == N.SI_OK
+    ref = idx[:total].clone()
+    # query q given one slot too many: its list leaves a gap
+    bad = off.clone()
+    bad[q + 1:] += 1
+    idx.fill_(-1)
+    ix.fill_planned(plan, nq, bad, idx, order=order)
+    assert N.lib.siPlanCheck(ix.handle, plan.data_ptr(), None) == N.SI_ERR_MISMATCH
+    # query q given one slot too few (q + 1 one more): q's list is cut, never overrun
+    bad = off.clone()
+    bad[q + 1] -= 1
+    idx.fill_(-1)
+    ix.fill_planned(plan, nq, bad, idx, order=order)
+    assert N.lib.siPlanCheck(ix.handle, plan.data_ptr(), None) == N.SI_ERR_MISMATCH
+    o0, o1 = int(off[q].item()), int(off[q + 1].item())
+    assert torch.equal(idx[o0:o1 - 1], ref[o0:o1 - 1])
+    # and a correct fill clears the report
+    ix.fill_planned(plan, nq, off, idx, order=order)
+    assert N.lib.siPlanCheck(ix.handle, plan.data_ptr(), None) == N.SI_OK
+    assert torch.equal(idx[:total], ref)
+    ix.close()
+
+
 def test_python_intervalmap_mirror(sia, orc):
     """The Python IntervalMap mirrors intervalmap.pyx on the reference's examples."""
     m = sia.IntervalMap()

@@ -15,6 +15,13 @@ is the reference's sweep over the oracle's descending search.
 import numpy as np
 import pytest
 
+import ctypes as C
+import os
+import subprocess
+import tempfile
+
+from conftest import GOLDEN, ROOT, overlap_pieces, sweep_difference
+
 pytestmark = pytest.mark.gpu
 
 
@@ -35,24 +42,6 @@ def _sets(seed, dtype, na=300, nb=400, span=3000):
     b_e = b_s + rng.integers(0, 60, nb)
     base = (np.int64(5) << 36) if dtype == np.int64 else 0
     return tuple((x + base).astype(dtype) for x in (a_s, a_e, b_s, b_e))
-
-
-def _reference_difference(a_s, a_e, ob_s, ob_e, hits_desc):
-    """The reference sweep (superintervals.hpp:1188-1211) over B's keys sorted ascending."""
-    out = []
-    for k in range(a_s.size):
-        cover = sorted((int(ob_s[h]), int(ob_e[h])) for h in hits_desc[k])
-        s, e = int(a_s[k]), int(a_e[k])
-        cursor = s
-        for cs0, ce0 in cover:
-            cs, ce = max(cs0, s), min(ce0, e)
-            if cs > cursor:
-                out.append((cursor, cs - 1, k))
-            if ce + 1 > cursor:
-                cursor = ce + 1
-        if cursor <= e:
-            out.append((cursor, e, k))
-    return out
 
 
 @pytest.mark.parametrize("dtype", [np.int32, np.int64])
@@ -108,7 +97,7 @@ def test_difference_pieces(sia, orc, dtype, seed):
     # order and pieces exactly as the reference's sweep
     ooff, oidx = orc.search_batch(oix, a_s, a_e, mode=0)
     hits = [oidx[ooff[k]:ooff[k + 1]] for k in range(a_s.size)]
-    ref = _reference_difference(a_s, a_e, oix["starts"], oix["ends"], hits)
+    ref = sweep_difference(a_s, a_e, oix["starts"], oix["ends"], hits)
     assert list(zip(ps.tolist(), pe.tolist(), pk.tolist())) == ref
     ix.close()
 
@@ -154,3 +143,115 @@ def test_intervalmap_set_ops(sia, orc):
     empty = IM.from_arrays([], [])
     assert [a.difference(empty).at(i) for i in range(4)] == [a.at(i) for i in range(4)]
     assert a.intersection(empty).size() == 0
+
+
+def _extras():
+    return dict(np.load(os.path.join(GOLDEN, "extras.npz")))
+
+
+def _capi_handle(N, s, e, d, index):
+    L = N.lib
+    h = L.createSuperIntervals()
+    for i in range(s.size):
+        L.addInterval(h, int(s[i]), int(e[i]), int(d[i]))
+    if index:
+        L.indexSuperIntervals(h)
+    return h
+
+
+def _capi_arrays(N, h):
+    st = N.cSuperIntervals.from_address(h)
+    n = st.size
+    if n == 0:
+        return np.zeros((3, 0), np.int64)
+    return np.stack([np.ctypeslib.as_array(st.starts, (n,)), np.ctypeslib.as_array(st.ends, (n,)),
+                     np.ctypeslib.as_array(st.data, (n,))]).astype(np.int64)
+
+
+def test_capi_intersection_difference_golden(sia):
+    """intersection / difference of the C-ABI (c_superintervals.h:286, :292) against the
+    reference C implementation's results; first operand in insertion order with 10 % of its
+    intervals inverted, the second indexed."""
+    import superintervals_amd._native as N
+    print("[gpu] C-ABI intersection / difference", flush=True)
+    x = _extras()
+    a_s, a_e = x["set_i32_a"]
+    b_s, b_e = x["set_i32_b"]
+    a = _capi_handle(N, a_s, a_e, x["set_i32_ad"], False)
+    b = _capi_handle(N, b_s, b_e, x["set_i32_bd"], True)
+    r = N.lib.intersection(a, b, None)
+    assert np.array_equal(_capi_arrays(N, r), x["set_i32_c_intersection"])
+    assert not N.cSuperIntervals.from_address(r).dev     # results are unindexed
+    N.lib.destroySuperIntervals(r)
+    r = N.lib.difference(a, b)
+    assert np.array_equal(_capi_arrays(N, r), x["set_i32_c_difference"])
+    N.lib.destroySuperIntervals(r)
+    # a combine callback sees (first operand's data, second operand's data)
+    CB = C.CFUNCTYPE(C.c_int32, C.c_int32, C.c_int32)
+    comb = CB(lambda p, q: p * 7 + q)
+    r = N.lib.intersection(a, b, C.cast(comb, C.c_void_p))
+    got = _capi_arrays(N, r)
+    N.lib.destroySuperIntervals(r)
+    sb = N.cSuperIntervals.from_address(b)
+    bd_sorted = np.ctypeslib.as_array(sb.data, (sb.size,)).astype(np.int64)
+    bs_sorted = np.ctypeslib.as_array(sb.starts, (sb.size,)).astype(np.int64)
+    be_sorted = np.ctypeslib.as_array(sb.ends, (sb.size,)).astype(np.int64)
+    res = N.cIndexResult()
+    hits = []
+    for k in range(a_s.size):
+        N.lib.clearIndexResult(C.byref(res))
+        N.lib.searchIdxs(b, int(a_s[k]), int(a_e[k]), C.byref(res))
+        hits.append([res.data[j] for j in range(res.size)])
+    N.lib.destroyIndexResult(C.byref(res))
+    want = [(s0, e0, int(x["set_i32_ad"][k]) * 7 + int(bd_sorted[h]))
+            for s0, e0, k, h in overlap_pieces(a_s, a_e, bs_sorted, be_sorted, hits)]
+    assert np.array_equal(got, np.array(want, np.int64).T)
+    # an empty second operand covers nothing; an unindexed one gives no result
+    e_ = N.lib.createSuperIntervals()
+    r = N.lib.difference(a, e_)
+    keep = a_s <= a_e
+    assert np.array_equal(_capi_arrays(N, r), np.stack([a_s[keep], a_e[keep], x["set_i32_ad"][keep]]).astype(np.int64))
+    N.lib.destroySuperIntervals(r)
+    N.lib.addInterval(e_, 1, 2, 3)
+    r = N.lib.intersection(a, e_, None)
+    assert N.cSuperIntervals.from_address(r).size == 0
+    assert b"indexSuperIntervals" in N.lib.siGetLastError()
+    for hh in (r, e_, a, b):
+        N.lib.destroySuperIntervals(hh)
+
+
+def test_python_and_cpp_set_ops_golden(sia):
+    """The Python IntervalMap (int32) and the C++ header (int32, int64) against the reference
+    C++ intersection / difference on the same inputs."""
+    print("[gpu] Python / C++ set operations against the reference", flush=True)
+    x = _extras()
+    a_s, a_e = x["set_i32_a"]
+    b_s, b_e = x["set_i32_b"]
+    IM = sia.IntervalMap
+    a = IM()
+    for s0, e0, d0 in zip(a_s.tolist(), a_e.tolist(), x["set_i32_ad"].tolist()):
+        a.add(s0, e0, d0)
+    b = IM.from_arrays(b_s, b_e, x["set_i32_bd"].tolist())
+    for op in ("intersection", "difference"):
+        r = getattr(a, op)(b)
+        got = np.array([r.at(i) for i in range(r.size())], np.int64).T
+        assert np.array_equal(got, x[f"set_i32_cpp_{op}"]), op
+    exe = os.path.join(ROOT, "tests", "cpp", "test_intervalmap")
+    for sfx, width in (("i32", 4), ("i64", 8)):
+        a_s, a_e = x[f"set_{sfx}_a"].astype(np.int64)
+        b_s, b_e = x[f"set_{sfx}_b"].astype(np.int64)
+        words = np.concatenate([[width, a_s.size], a_s, a_e, x[f"set_{sfx}_ad"], [b_s.size], b_s, b_e,
+                                x[f"set_{sfx}_bd"]]).astype(np.int64)
+        with tempfile.TemporaryDirectory() as d:
+            fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+            words.tofile(fin)
+            rr = subprocess.run([exe, "setops", fin, fout], capture_output=True, text=True, timeout=240)
+            assert rr.returncode == 0, rr.stderr[-2000:]
+            w = np.fromfile(fout, np.int64)
+        ni = int(w[0])
+        inter = w[1:1 + 3 * ni].reshape(3, ni)
+        nd = int(w[1 + 3 * ni])
+        diff = w[2 + 3 * ni:2 + 3 * ni + 3 * nd].reshape(3, nd)
+        # the C++ header drops empty intersection pieces on the host
+        assert np.array_equal(inter, x[f"set_{sfx}_cpp_intersection"]), sfx
+        assert np.array_equal(diff, x[f"set_{sfx}_cpp_difference"]), sfx
