@@ -57,7 +57,10 @@ def parse():
                    help="small batches: eager plan/offsets/fill vs one captured graph replay")
     p.add_argument("--graph-nq", type=int, default=4096)
     p.add_argument("--walk-once", action="store_true",
-                   help="plan the 10M x 10M batch, then one count walk and one fill walk (for PMC counters)")
+                   help="one launch of each kernel of the 10M x 10M step (plan, offsets, rank count, count walk, "
+                        "fill) with the --tuning knobs, for PMC counters")
+    p.add_argument("--tuning", type=str, default=None,
+                   help="JSON object of launch knobs to apply instead of autotuning (e.g. the 'launch' of a run)")
     return p.parse_args()
 
 
@@ -221,7 +224,7 @@ def cpu_baseline(s, e, qs, qe, sample, check=None):
 
 def verify_output(ix, tqs, tqe, s, e, off, idx, nq, stream):
     """Checks the timed configuration's own output on the device: counts against the closed
-    form (torch.searchsorted over the sorted starts and ends, every query has s <= e), the
+    form (torch.searchsorted over the starts and the sorted ends, every query has s <= e), the
     walk-form count against the offsets (two different device algorithms), every emitted
     position overlaps its query, every list strictly descending.  Returns a dict of results."""
     import torch
@@ -248,7 +251,10 @@ def verify_output(ix, tqs, tqe, s, e, off, idx, nq, stream):
     total = int(off[-1].item())
     qid = torch.repeat_interleave(torch.arange(nq, device=dev, dtype=torch.int32), cnt)
     pos = idx[:total].to(torch.int64)
-    ends_t = torch.from_numpy(e).to(dev)
+    # positions index the built (sorted) arrays: equal starts are ordered by end descending
+    arr = ix.download()
+    ss = torch.from_numpy(arr["starts"]).to(dev)
+    ends_t = torch.from_numpy(arr["ends"]).to(dev)
     ok = bool((ss[pos] <= tqe[qid]).all()) and bool((ends_t[pos] >= tqs[qid]).all())
     res["positions_overlap_query"] = ok
     same = qid[1:] == qid[:-1]
@@ -528,14 +534,33 @@ def bench_graph(a):
     print(json.dumps(out))
 
 
+def tuning_arg(a):
+    """The --tuning knobs (launch-choice keys only), or None."""
+    if not a.tuning:
+        return None
+    from superintervals_amd import autotune
+    knobs = json.loads(a.tuning)
+    return {k: v for k, v in knobs.items() if k in autotune.KNOBS}
+
+
 def walk_once(a):
-    """One count walk and one descending fill walk over the planned 10M x 10M batch."""
+    """One launch of each kernel of the 10M x 10M step with the --tuning knobs: plan, offsets
+    (rank count + scan), the rank count alone, the count walk, the descending fill."""
     import numpy as np
     import torch
     import superintervals_amd as sia
     from superintervals_amd import autotune
     dev = torch.device("cuda:0")
     torch.cuda.init()
+    if a.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.int32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    knobs = tuning_arg(a)
+    if knobs:
+        autotune.apply(knobs)
     s, e = sia.workload_intervals(2, a.n, seed=1, sorted=True)
     qs, qe = sia.workload_queries(2, a.nq, seed=2)
     ix = sia.DeviceIndex(np.int32).build(s, e)
@@ -548,11 +573,13 @@ def walk_once(a):
     ix.offsets_planned(plan, nq, off)
     torch.cuda.synchronize()
     idx = torch.empty(int(off[-1].item()), dtype=torch.int32, device=dev)
+    autotune.apply({"count_rank": 1})
+    ix.count_planned(plan, nq, counts)
     autotune.apply({"count_rank": 0})
     ix.count_planned(plan, nq, counts)
     ix.fill_planned(plan, nq, off, idx)
     torch.cuda.synchronize()
-    print(json.dumps({"walk_once": True, "queries": nq, "hits": int(off[-1].item())}))
+    print(json.dumps({"walk_once": True, "queries": nq, "hits": int(off[-1].item()), "tuning": autotune.current()}))
 
 
 def main():
@@ -603,7 +630,11 @@ def main():
 
     # launch choices: autotuned on this index and batch before the timed region
     tuning = None
-    if not a.no_tune:
+    if tuning_arg(a):
+        from superintervals_amd import autotune
+        tuning = tuning_arg(a)
+        autotune.apply(tuning)
+    elif not a.no_tune:
         from superintervals_amd import autotune
         tuning = autotune.load_or_tune(ix, plan, nq, off, idx, total, stream=stream, retune=True,
                                        qs_t=tqs, qe_t=tqe, verbose=a.verbose)
