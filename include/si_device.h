@@ -185,6 +185,25 @@ int siSearchFillPlannedBounded(const siIndex* ix, void* plan, size_t nq, int ord
 int siSearchValuesFillPlannedBounded(const siIndex* ix, void* plan, size_t nq, const uint64_t* offsets,
                                      int32_t* values, uint64_t capacity, void* stream);
 
+/* ---- Search without allocation ---------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * two-pass search of siSearchOffsets + siSearchFillPlannedBounded in one call, with the
+ * batch's plan kept in a static device array of the library instead of allocated memory
+ * (56 MB for the 2^20-query maximum, held on every device the library runs on until the
+ * process ends).  Calls no hipMalloc/hipFree and no allocating library routine, and reads
+ * nothing back to the host, so it can be captured in a graph.
+ *   nq <= SI_NOALLOC_MAX_QUERIES, else SI_ERR_ARG.
+ *   counts (may be NULL) and offsets[nq + 1] as siSearchOffsets; with capacity > 0 the
+ *   positions in the given order go to idx[0 .. capacity) as siSearchFillPlannedBounded
+ *   (offsets[nq] > capacity means the lists were cut); capacity 0 skips the fill.
+ * Every call on one device shares that device's array and leaves its contents behind:
+ * calls that overlap on one device (concurrent streams or threads) race and must be
+ * serialised by the caller.  Results equal siSearchOffsets + siSearchFill. */
+#define SI_NOALLOC_MAX_QUERIES (1u << 20)
+int siSearchNoAlloc(const siIndex* ix, const void* qs, const void* qe, size_t nq, int order,
+                    uint32_t* counts, uint64_t* offsets, uint32_t* idx, uint64_t capacity, void* stream);
+
 /* ---- Batched queries on device buffers ------------------------------------
  * These plan the batch internally (stream-ordered scratch) and then run. */
 

@@ -86,6 +86,7 @@ _SIGS = {
     "siSearchOffsetsPlanned": (_i, [_vp, _vp, _sz, _vp, _vp, _vp]),
     "siSearchFillPlanned": (_i, [_vp, _vp, _sz, _i, _vp, _vp, _vp]),
     "siPlanCheck": (_i, [_vp, _vp, _vp]),
+    "siSearchNoAlloc": (_i, [_vp, _vp, _vp, _sz, _i, _vp, _vp, _vp, C.c_uint64, _vp]),
     "siSearchValuesFillPlanned": (_i, [_vp, _vp, _sz, _vp, _vp, _vp]),
     "siSearchFillPlannedBounded": (_i, [_vp, _vp, _sz, _i, _vp, _vp, C.c_uint64, _vp]),
     "siSearchValuesFillPlannedBounded": (_i, [_vp, _vp, _sz, _vp, _vp, C.c_uint64, _vp]),

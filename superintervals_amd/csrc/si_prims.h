@@ -199,7 +199,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const TI* in, size_t
 // Exclusive scan of n items (TI) into out (TO); out may alias in when TI == TO.
 // If total != nullptr (device pointer) the grand total is stored there.
 // ws must hold at least scan_ws_items(n) TO values.
-inline size_t scan_ws_items(size_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+constexpr size_t scan_ws_items(size_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
 
 template <typename TI, typename TO>
 int device_excl_scan(const TI* in, size_t n, TO* out, TO* total, TO* ws, hipStream_t st) {
@@ -347,11 +347,11 @@ struct RsTemp {
     uint32_t* ws;      // scan workspace
 };
 
-inline size_t rs_ntiles(size_t n) { return (n + RS_TILE - 1) / RS_TILE; }
+constexpr size_t rs_ntiles(size_t n) { return (n + RS_TILE - 1) / RS_TILE; }
 
-inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-inline size_t rs_temp_bytes(size_t n) {
+constexpr size_t rs_temp_bytes(size_t n) {
     const size_t nth = rs_ntiles(n) * RS_BINS;
     return align256(8 * RS_BINS * 4) + 2 * align256(nth * 4) + align256(scan_ws_items(nth) * 4);
 }

@@ -34,6 +34,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -82,7 +83,8 @@ struct HeavyEntry {
 
 struct PlanLayout {
     size_t off_q, off_s, off_e, off_idx, off_scratch, off_k0, off_k1, off_v0, off_v1, off_rs, off_heavy, total;
-    PlanLayout(int cb, size_t nq) {
+    constexpr PlanLayout(int cb, size_t nq) : off_q(0), off_s(0), off_e(0), off_idx(0), off_scratch(0), off_k0(0),
+                                             off_k1(0), off_v0(0), off_v1(0), off_rs(0), off_heavy(0), total(0) {
         off_q = 256;
         off_s = off_q + align256(4 * nq);
         off_e = off_s + align256((size_t)cb * nq);
@@ -238,7 +240,7 @@ struct Tuning {
 Tuning g_tune;
 
 template <typename S>
-int plan_impl(const siIndex* ix, const S* qs, const S* qe, size_t nq, void* plan, hipStream_t st) {
+int plan_impl(const siIndex* ix, const S* qs, const S* qe, size_t nq, void* plan, hipStream_t st, bool read_back) {
     PlanLayout L(sizeof(S), nq);
     char* p = (char*)plan;
     uint64_t* k0 = (uint64_t*)(p + L.off_k0);
@@ -253,11 +255,11 @@ int plan_impl(const siIndex* ix, const S* qs, const S* qe, size_t nq, void* plan
     if (sizeof(S) == 4) {
         k_plan_keys_i32<<<g, Q_THREADS, 0, st>>>((const int32_t*)qs, (const int32_t*)qe, nq, k0, v0);
         LAUNCH_CHECK();
-        rc = radix_sort_pairs(k0, v0, k1, v1, nq, 4 + g_tune.plan_skip_low, 8, rt, &ks, &vs, st, g_tune.plan_sync != 0);
+        rc = radix_sort_pairs(k0, v0, k1, v1, nq, 4 + g_tune.plan_skip_low, 8, rt, &ks, &vs, st, read_back);
     } else {
         k_plan_keys_i64<<<g, Q_THREADS, 0, st>>>((const int64_t*)qe, nq, k0, v0);
         LAUNCH_CHECK();
-        rc = radix_sort_pairs(k0, v0, k1, v1, nq, g_tune.plan_skip_low, 8, rt, &ks, &vs, st, g_tune.plan_sync != 0);
+        rc = radix_sort_pairs(k0, v0, k1, v1, nq, g_tune.plan_skip_low, 8, rt, &ks, &vs, st, read_back);
     }
     if (rc != SI_OK) return rc;
     // upper_bound of the sorted ends: through the search tree when the index has one
@@ -1226,7 +1228,7 @@ SI_EXPORT int siPlanQueries(const siIndex* ix, const void* qs, const void* qe, s
     if (nq == 0) return SI_OK;
     if (!qs || !qe) return si_set_error(SI_ERR_ARG, "null queries");
     hipStream_t st = (hipStream_t)stream;
-    return DISPATCH_S(ix, plan_impl<S>(ix, (const S*)qs, (const S*)qe, nq, plan, st));
+    return DISPATCH_S(ix, plan_impl<S>(ix, (const S*)qs, (const S*)qe, nq, plan, st, g_tune.plan_sync != 0));
 }
 
 SI_EXPORT int siCountPlanned(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, void* stream) {
@@ -1598,6 +1600,75 @@ SI_EXPORT int siWalkStats(const siIndex* ix, const void* qs, const void* qe, siz
         LAUNCH_CHECK();
         return SI_OK;
     }()));
+}
+
+// ============================================================================
+// Search without allocation
+// ============================================================================
+// siSearchNoAlloc keeps its plan in a file-scope device array instead of memory it
+// allocates: sized for SI_NOALLOC_MAX_QUERIES (si_device.h) queries of either coordinate
+// width (int64 plans are the larger), held by every device the library runs on until the process ends.
+// In host code `scratch` names a host-side stand-in; its device address, one per device,
+// comes from hipGetSymbolAddress on the first call on that device.
+static constexpr size_t SI_NOALLOC_BYTES = PlanLayout(8, SI_NOALLOC_MAX_QUERIES).total;
+static_assert(PlanLayout(4, SI_NOALLOC_MAX_QUERIES).total <= SI_NOALLOC_BYTES, "int32 plans fit");
+static __device__ __align__(256) unsigned char scratch[SI_NOALLOC_BYTES];
+
+namespace {
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[SCRATCH_DEVICES] = {nullptr};
+
+// Device address of `scratch` on the current device (looked up once per device).
+int scratch_ptr(void** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_DEVICES) return si_set_error(SI_ERR_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return si_set_error(SI_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SI_OK;
+}
+
+__global__ void k_zero_u64(uint64_t* __restrict__ p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+}  // namespace
+
+SI_EXPORT int siSearchNoAlloc(const siIndex* ix, const void* qs, const void* qe, size_t nq, int order,
+                              uint32_t* counts, uint64_t* offsets, uint32_t* idx, uint64_t capacity, void* stream) {
+    int rc = check_ix(ix);
+    if (rc) return rc;
+    if (order != SI_ORDER_DESC && order != SI_ORDER_VECTOR) return si_set_error(SI_ERR_ARG, "bad order %d", order);
+    if (nq > SI_NOALLOC_MAX_QUERIES)
+        return si_set_error(SI_ERR_ARG, "siSearchNoAlloc takes at most %u queries per call (got %zu)",
+                            (unsigned)SI_NOALLOC_MAX_QUERIES, nq);
+    if (!offsets || (nq && (!qs || !qe)) || (capacity && !idx)) return si_set_error(SI_ERR_ARG, "null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    if (nq == 0) {
+        k_zero_u64<<<1, 64, 0, st>>>(offsets, 1);
+        LAUNCH_CHECK();
+        return SI_OK;
+    }
+    void* plan = nullptr;
+    if ((rc = scratch_ptr(&plan))) return rc;
+    // no host read-back in the plan, so the call can also be captured in a graph
+    rc = DISPATCH_S(ix, plan_impl<S>(ix, (const S*)qs, (const S*)qe, nq, plan, st, false));
+    if (rc) return rc;
+    if ((rc = siSearchOffsetsPlanned(ix, plan, nq, counts, offsets, stream))) return rc;
+    if (capacity == 0) return SI_OK;
+    if (order == SI_ORDER_DESC)
+        return DISPATCH_S(ix, (launch_walk<S, W_DESC>(ix, plan, nq, nullptr, nullptr, offsets, idx, nullptr, st,
+                                                      capacity)));
+    return DISPATCH_S(ix, (launch_walk<S, W_VECTOR>(ix, plan, nq, nullptr, nullptr, offsets, idx, nullptr, st,
+                                                    capacity)));
 }
 
 // ============================================================================
