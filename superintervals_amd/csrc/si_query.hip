@@ -56,16 +56,18 @@ struct HeavyEntry;
 template <typename S>
 struct PlanView {
     uint32_t* counter;   // [0] tiles taken, [1] heavy entries written, [2] heavy entries taken,
-                         // [3] set by a fill pass when a query's hits differ from its offsets
+                         // [3] set by a fill pass when a query's hits differ from its offsets,
+                         // [4] 1 when the plan holds lower_bound positions (lb)
     uint32_t* q;         // input query number of sorted position k
     S* s;                // query start of sorted position k
     S* e;                // query end of sorted position k
     uint32_t* idx;       // upper_bound of sorted position k
+    uint32_t* lb;        // first position whose start is >= the query start (lower_bound)
     HeavyEntry* heavy;   // parked walks (see HeavyEntry)
 };
 
 // Byte layout of a plan of nq queries (coordinate width cb):
-//   [256 B counters][q: 4 nq][s: cb nq][e: cb nq][idx: 4 nq][scratch: keys 2 x 8 nq,
+//   [256 B counters][q: 4 nq][s: cb nq][e: cb nq][idx: 4 nq][lb: 4 nq][scratch: keys 2 x 8 nq,
 //   vals 2 x 4 nq, radix temporaries]; the scratch is reused as the offsets scan workspace.
 // A query whose walk exceeds the per-lane step budget, parked for the wave-cooperative
 // kernel: sorted position, current position, leading-run top and phase, the count so far
@@ -82,14 +84,15 @@ struct HeavyEntry {
 };
 
 struct PlanLayout {
-    size_t off_q, off_s, off_e, off_idx, off_scratch, off_k0, off_k1, off_v0, off_v1, off_rs, off_heavy, total;
-    constexpr PlanLayout(int cb, size_t nq) : off_q(0), off_s(0), off_e(0), off_idx(0), off_scratch(0), off_k0(0),
+    size_t off_q, off_s, off_e, off_idx, off_lb, off_scratch, off_k0, off_k1, off_v0, off_v1, off_rs, off_heavy, total;
+    constexpr PlanLayout(int cb, size_t nq) : off_q(0), off_s(0), off_e(0), off_idx(0), off_lb(0), off_scratch(0), off_k0(0),
                                              off_k1(0), off_v0(0), off_v1(0), off_rs(0), off_heavy(0), total(0) {
         off_q = 256;
         off_s = off_q + align256(4 * nq);
         off_e = off_s + align256((size_t)cb * nq);
         off_idx = off_e + align256((size_t)cb * nq);
-        off_scratch = off_idx + align256(4 * nq);
+        off_lb = off_idx + align256(4 * nq);
+        off_scratch = off_lb + align256(4 * nq);
         off_k0 = off_scratch;
         off_k1 = off_k0 + align256(8 * nq);
         off_v0 = off_k1 + align256(8 * nq);
@@ -113,6 +116,7 @@ PlanView<S> plan_view(void* base, size_t nq) {
     v.s = (S*)(p + L.off_s);
     v.e = (S*)(p + L.off_e);
     v.idx = (uint32_t*)(p + L.off_idx);
+    v.lb = (uint32_t*)(p + L.off_lb);
     v.heavy = (HeavyEntry*)(p + L.off_heavy);
     return v;
 }
@@ -138,13 +142,21 @@ __global__ void k_plan_keys_i64(const int64_t* __restrict__ qe, size_t nq, uint6
     }
 }
 
+// lower_bound from upper_bound: the first position whose start is >= s is upper_bound(s - 1)
+// + 1 (NONE + 1 wraps to 0), and 0 when s is the smallest value.
+template <typename S>
+__device__ __forceinline__ S smin_of() { return (sizeof(S) == 4) ? (S)INT32_MIN : (S)INT64_MIN; }
+
 template <typename S>
 __global__ __launch_bounds__(Q_THREADS) void k_plan_finish(IndexView<S> ix, const uint64_t* __restrict__ key,
                                                            const uint32_t* __restrict__ val,
                                                            const S* __restrict__ qs, size_t nq,
-                                                           PlanView<S> pv, bool with_ub) {
+                                                           PlanView<S> pv, bool with_ub, bool with_lb) {
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == 0) pv.counter[0] = 0;
+    if (k == 0) {
+        pv.counter[0] = 0;
+        pv.counter[4] = with_lb ? 1u : 0u;
+    }
     if (k >= nq) return;
     uint64_t kk = key[k];
     uint32_t q;
@@ -161,13 +173,20 @@ __global__ __launch_bounds__(Q_THREADS) void k_plan_finish(IndexView<S> ix, cons
     pv.q[k] = q;
     pv.s[k] = s;
     pv.e[k] = e;
-    if (with_ub) pv.idx[k] = upper_bound_pos(ix.starts, ix.n, e);
+    if (with_ub) {
+        pv.idx[k] = upper_bound_pos(ix.starts, ix.n, e);
+        if (with_lb) pv.lb[k] = (s == smin_of<S>()) ? 0u : upper_bound_pos(ix.starts, ix.n, (S)(s - 1)) + 1u;
+    }
 }
 
 // upper_bound of n keys through the index's search tree over the starts (defined with its
 // kernel below).
 template <typename S>
 int launch_ub_tree(const siIndex* ix, const S* xs, size_t n, uint32_t* out, hipStream_t st);
+
+// The plan's upper_bound (and lower_bound) positions through the same tree (below).
+template <typename S>
+int launch_plan_bounds(const siIndex* ix, PlanView<S> pv, size_t nq, bool with_lb, hipStream_t st);
 
 // Number of sorted ends below x (branchless lower_bound).
 template <typename S>
@@ -225,6 +244,9 @@ __global__ __launch_bounds__(Q_THREADS) void k_count_rank(IndexView<S> ix, const
 //                one radix pass fewer each; queries are then in end order only up to
 //                those bits, which the walk's tiles allow (their LDS window ends at the
 //                tile's largest upper_bound) at some cost in window hits
+//   range_skip   1: the plan also finds each query's lower_bound and the walks take the
+//                run [lower_bound, upper_bound] (every position there starts inside the
+//                query) as hits without reading ends; 0: they walk it
 struct Tuning {
     uint32_t tile = 1024;
     int count_wgs = 0;
@@ -236,8 +258,13 @@ struct Tuning {
     int ub_tree = 1;
     int plan_skip_low = 0;
     int heavy_ends32 = 1;
+    int range_skip = 1;
 };
 Tuning g_tune;
+
+// The run [lower_bound(s), upper_bound(e)] holds only hits when every interval has
+// start <= end (each starts at or after s, so ends at or after s).
+inline bool range_skip_on(const siIndex* ix) { return g_tune.range_skip && ix->n_inverted == 0; }
 
 template <typename S>
 int plan_impl(const siIndex* ix, const S* qs, const S* qe, size_t nq, void* plan, hipStream_t st, bool read_back) {
@@ -262,12 +289,14 @@ int plan_impl(const siIndex* ix, const S* qs, const S* qe, size_t nq, void* plan
         rc = radix_sort_pairs(k0, v0, k1, v1, nq, g_tune.plan_skip_low, 8, rt, &ks, &vs, st, read_back);
     }
     if (rc != SI_OK) return rc;
-    // upper_bound of the sorted ends: through the search tree when the index has one
+    // upper_bound of the sorted ends (and lower_bound of the starts when the walks take the
+    // guaranteed run): through the search tree when the index has one
     const bool tree = g_tune.ub_tree && ix->ub.d_top && ix->n > 0;
+    const bool with_lb = range_skip_on(ix);
     const PlanView<S> pv = plan_view<S>(plan, nq);
-    k_plan_finish<S><<<g, Q_THREADS, 0, st>>>(ix->view<S>(), ks, vs, qs, nq, pv, !tree);
+    k_plan_finish<S><<<g, Q_THREADS, 0, st>>>(ix->view<S>(), ks, vs, qs, nq, pv, !tree, with_lb);
     LAUNCH_CHECK();
-    if (tree) return launch_ub_tree<S>(ix, pv.e, nq, pv.idx, st);
+    if (tree) return launch_plan_bounds<S>(ix, pv, nq, with_lb, st);
     return SI_OK;
 }
 
@@ -308,6 +337,8 @@ __device__ __forceinline__ uint32_t miss_top(uint32_t kc, uint32_t m) {
 
 // One slot of a wave's emission scratch in LDS: the segment that starts at this output
 // slot of the current 64-slot chunk.  kv = m | kind << 16 | 1 << 31 (valid); 0 when free.
+// kind bit 0: ascending range base .. base + c - 1; bit 1: descending range base ..
+// base - c + 1; neither: the set bits of m, highest first, offset by base.
 struct __align__(8) EmitSlot {
     uint32_t pre, kv, base, c;
     uint32_t o_lo, o_hi;
@@ -357,9 +388,10 @@ __device__ __forceinline__ void put_segment(EmitSlot* __restrict__ sd, uint32_t 
 }
 
 // One output segment of a lane: c values at output offset o, either the ascending range
-// base .. base + c - 1 (asc) or the set bits of m (descending) offset by base.
+// base .. base + c - 1 (asc), the descending range base .. base - c + 1 (desc), or the set
+// bits of m (descending) offset by base.
 struct Seg {
-    uint32_t c, mk, base;   // mk = m | asc << 16
+    uint32_t c, mk, base;   // mk = m | asc << 16 | desc << 17
     uint64_t o;
 };
 
@@ -410,6 +442,8 @@ __device__ __forceinline__ void emit_segments(const Seg (&sg)[N], int lane, Emit
         uint32_t pos;
         if ((d.kv >> 16) & 1u) {
             pos = d.base + r;
+        } else if ((d.kv >> 17) & 1u) {
+            pos = d.base - r;
         } else if (SI_EMIT_LUT) {
             // the (r+1)-th highest set bit of om, from the byte table (r >= c only happens
             // on slots that store nothing)
@@ -467,6 +501,7 @@ constexpr int T_THREADS = 512;   // 8 waves per workgroup, two workgroups per CU
 template <typename S, int MODE>
 __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<S> ix, PlanView<S> pv, uint32_t nq,
                                                     uint32_t tile, uint32_t ntiles, uint32_t lane_steps,
+                                                    bool use_lb,
                                                     uint32_t* __restrict__ tile_counter,
                                                     uint32_t* __restrict__ counts,
                                                     uint64_t* __restrict__ counts64,
@@ -484,6 +519,8 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
     __shared__ uint32_t s_kbit[(MODE == W_COUNT || !SI_EMIT_LUT) ? 1 : 256];   // byte set-bit table
     const int lane = lane_id();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // the guaranteed runs need the plan's lower_bound positions
+    use_lb = use_lb && __builtin_amdgcn_readfirstlane(pv.counter[4]) != 0u;
     EmitSlot* const my_emit = s_emit + (MODE == W_COUNT ? 0u : (threadIdx.x / WAVE) * (WAVE + 1));
     if (MODE != W_COUNT) my_emit[lane].kv = 0u;   // emission slots start free (own wave's)
     // segments held from the previous steps (emission runs every EMIT_EVERY steps)
@@ -537,6 +574,7 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
         uint32_t pool_base = 0, pool_left = 0;
         bool exhausted = false, has = false, lead = false;
         uint32_t q = 0, cur = NONE, cnt = 0, top = 0, kq = 0, steps = 0;
+        uint32_t rng = NONE;   // lower end of the query's guaranteed run [rng, cur], or NONE
         S s = 0;
         uint64_t o = 0, oend = 0;   // output cursor and end of the query's CSR segment
         while (true) {
@@ -569,6 +607,10 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
                         oend = min(o1, cap);
                     }
                     if (MODE == W_VECTOR) { top = cur; lead = true; }
+                    if (use_lb) {
+                        const uint32_t b = pv.lb[k];
+                        rng = (cur != NONE && b <= cur) ? b : NONE;
+                    }
                 }
                 const uint32_t take = min((uint32_t)__popcll(nm), pool_left);
                 pool_base += take;
@@ -582,9 +624,35 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
             // ascending range e_base .. e_base + c_emit - 1 or the set bits of e_m
             // (descending) offset by e_base
             uint32_t c_emit = 0, e_m = 0, e_base = 0;
-            bool e_asc = false;
+            bool e_asc = false, e_desc = false;
             uint64_t e_o = 0;
-            if (has && cur != NONE) {
+            if (has && rng != NONE) {
+                // [rng, cur] = positions starting inside the query (lower_bound .. upper_bound):
+                // all hits, taken without reading ends; the walk goes on below them
+                const uint32_t c = cur - rng + 1u;
+                if (MODE == W_COUNT) {
+                    cnt += c;
+                } else if (MODE != W_VECTOR) {
+                    cnt -= c;
+                    c_emit = (uint32_t)min((uint64_t)c, oend - o);
+                    e_base = cur;
+                    e_desc = true;
+                    e_o = o;
+                    o += c_emit;
+                }
+                cur = rng - 1u;   // rng == 0 wraps to NONE
+                rng = NONE;
+                if (MODE == W_VECTOR && cur == NONE) {
+                    // the leading run reaches position 0: written ascending now
+                    cnt -= top + 1u;
+                    c_emit = (uint32_t)min((uint64_t)(top + 1u), oend - o);
+                    e_base = 0;
+                    e_asc = true;
+                    e_o = o;
+                    o += c_emit;
+                    lead = false;
+                }
+            } else if (has && cur != NONE) {
                 const uint32_t b0 = cur & ~(uint32_t)(W - 1);
                 const uint32_t kc = cur - b0;
                 const bool in_win = b0 >= wlo;   // every position visited is <= whi
@@ -641,7 +709,7 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
             if (MODE != W_COUNT) {
                 // hold this step's segment; every EMIT_EVERY steps emit the held ones with
                 // it (phase is wave-uniform: every lane runs every step)
-                const Seg now = Seg{c_emit, e_m | ((e_asc ? 1u : 0u) << 16), e_base, e_o};
+                const Seg now = Seg{c_emit, e_m | ((e_asc ? 1u : 0u) << 16) | ((e_desc ? 1u : 0u) << 17), e_base, e_o};
                 if (phase == EMIT_EVERY - 1) {
                     Seg all[EMIT_EVERY];
 #pragma unroll
@@ -966,6 +1034,37 @@ int launch_ub_tree(const siIndex* ix, const S* xs, size_t n, uint32_t* out, hipS
     return SI_OK;
 }
 
+// The plan's bounds through the tree: idx[k] = upper_bound(e[k]) and, with_lb, lb[k] = the
+// first position whose start is >= s[k].  One thread per sorted query, two descents.
+template <typename S>
+__global__ __launch_bounds__(Q_THREADS) void k_plan_bounds_tree(UbTree<S> t, const S* __restrict__ top, uint32_t h,
+                                                                PlanView<S> pv, size_t nq, bool with_lb) {
+    extern __shared__ __align__(16) char s_mem[];
+    S* s_top = reinterpret_cast<S*>(s_mem);
+    const uint32_t T = (1u << h) - 1u;
+    for (uint32_t k = threadIdx.x; k < T; k += Q_THREADS) s_top[k] = top[k];
+    __syncthreads();
+    for (size_t k0 = (size_t)blockIdx.x * Q_THREADS; k0 < nq; k0 += (size_t)gridDim.x * Q_THREADS) {
+        const size_t k = k0 + threadIdx.x;
+        if (k >= nq) continue;
+        pv.idx[k] = ub_tree_pos(t, s_top, h, pv.e[k]);
+        if (with_lb) {
+            const S s = pv.s[k];
+            pv.lb[k] = (s == smin_of<S>()) ? 0u : ub_tree_pos(t, s_top, h, (S)(s - 1)) + 1u;
+        }
+    }
+}
+
+template <typename S>
+int launch_plan_bounds(const siIndex* ix, PlanView<S> pv, size_t nq, bool with_lb, hipStream_t st) {
+    const KeyTree& kt = ix->ub;
+    const size_t lds = (size_t)((1u << kt.top_h) - 1u) * ix->cb;
+    k_plan_bounds_tree<S><<<tree_grid(nq), Q_THREADS, lds, st>>>(ub_tree_of<S>(kt, (const S*)ix->d_starts, ix->n),
+                                                                  (const S*)kt.d_top, kt.top_h, pv, nq, with_lb);
+    LAUNCH_CHECK();
+    return SI_OK;
+}
+
 template <typename S>
 __global__ __launch_bounds__(Q_THREADS) void k_has_overlaps(IndexView<S> ix, const S* __restrict__ qs,
                                                             const S* __restrict__ qe, size_t nq,
@@ -1112,8 +1211,8 @@ int launch_walk(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint
     k_zero_u32<<<1, 64, 0, st>>>(pv.counter, 4);
     LAUNCH_CHECK();
     k_walk<S, MODE><<<grid, T_THREADS, 0, st>>>(
-        ix->view<S>(), pv, (uint32_t)nq, tile, ntiles, g_tune.lane_steps, pv.counter, counts, counts64,
-        offsets, out_idx, ix->d_values, out_val, cap);
+        ix->view<S>(), pv, (uint32_t)nq, tile, ntiles, g_tune.lane_steps, range_skip_on(ix), pv.counter, counts,
+        counts64, offsets, out_idx, ix->d_values, out_val, cap);
     LAUNCH_CHECK();
     // parked (heavy) walks: one wave each, resident grid, exits at once when none were parked
     static int cus = 0;
@@ -1185,6 +1284,9 @@ SI_EXPORT int siTuneSet(const char* name, int64_t value) {
     } else if (k == "heavy_ends32") {
         if (value != 0 && value != 1) return si_set_error(SI_ERR_ARG, "heavy_ends32 must be 0 or 1");
         g_tune.heavy_ends32 = (int)value;
+    } else if (k == "range_skip") {
+        if (value != 0 && value != 1) return si_set_error(SI_ERR_ARG, "range_skip must be 0 or 1");
+        g_tune.range_skip = (int)value;
     } else if (k == "plan_skip_low") {
         if (value < 0 || value > 2) return si_set_error(SI_ERR_ARG, "plan_skip_low must be in [0, 2]");
         g_tune.plan_skip_low = (int)value;
@@ -1208,6 +1310,7 @@ SI_EXPORT int64_t siTuneGet(const char* name) {
     if (k == "ub_tree") return g_tune.ub_tree;
     if (k == "plan_skip_low") return g_tune.plan_skip_low;
     if (k == "heavy_ends32") return g_tune.heavy_ends32;
+    if (k == "range_skip") return g_tune.range_skip;
     if (k == "count_rank") return g_tune.count_rank;
     if (k == "offsets_rank") return g_tune.offsets_rank;
     return -1;

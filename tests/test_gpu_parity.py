@@ -242,12 +242,13 @@ def test_launch_choices_do_not_change_results(sia, orc):
     ix = _dev_index(sia, s, e)
     saved = autotune.current()
     try:
-        for tile, cw, fw, cr, orank, psl in ((64, 1, 1, 0, 1, 0), (512, 0, 1, 1, 0, 1), (4096, 1, 0, 1, 1, 2),
-                                             (100000, 0, 0, 0, 0, 1), (1024, 0, 0, 1, 1, 2)):
-            print(f"[gpu] launch tile={tile} count_wgs={cw} fill_wgs={fw} rank={cr}/{orank} skip_low={psl}",
-                  flush=True)
+        for tile, cw, fw, cr, orank, psl, rs in ((64, 1, 1, 0, 1, 0, 1), (512, 0, 1, 1, 0, 1, 0),
+                                                 (4096, 1, 0, 1, 1, 2, 1), (100000, 0, 0, 0, 0, 1, 0),
+                                                 (1024, 0, 0, 1, 1, 2, 1), (1024, 0, 0, 0, 0, 0, 0)):
+            print(f"[gpu] launch tile={tile} count_wgs={cw} fill_wgs={fw} rank={cr}/{orank} skip_low={psl} "
+                  f"range_skip={rs}", flush=True)
             autotune.apply({"tile": tile, "count_wgs": cw, "fill_wgs": fw, "count_rank": cr,
-                            "offsets_rank": orank, "plan_skip_low": psl})
+                            "offsets_rank": orank, "plan_skip_low": psl, "range_skip": rs})
             assert np.array_equal(ix.count(qs, qe), rec["counts"])
             off, idx = ix.search(qs, qe, order="vector")
             assert orc.digest(orc.hash_csr(off, idx)) == int(rec["hash_vec_digest"])
@@ -388,6 +389,47 @@ def test_workload3_wide_queries(sia, orc):
         ooff, oidx = orc.search_batch(oix, qs, qe, mode=mode)
         assert np.array_equal(off, ooff)
         assert np.array_equal(idx.astype(np.int64), oidx)
+    ix.close()
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_wide_queries_guaranteed_run(sia, orc, dtype):
+    """Queries much wider than the intervals: most hits are the run [lower_bound(s),
+    upper_bound(e)] taken without reading ends.  Every mode and order, the run taken or
+    walked, lane and wave walks, against the oracle; some runs reach position 0."""
+    from superintervals_amd import autotune
+    print(f"[gpu] wide queries, guaranteed run dtype={np.dtype(dtype).name}", flush=True)
+    rng = np.random.default_rng(77)
+    n = 150_000
+    s = rng.integers(0, 3_000_000, n)
+    e = s + rng.integers(0, 200, n)
+    long_ = rng.random(n) < 0.02
+    e = np.where(long_, s + rng.integers(1000, 200_000, n), e)
+    qs = rng.integers(-5_000, 3_000_000, 6000)
+    qe = qs + rng.integers(20_000, 400_000, qs.size)
+    qs[:50] = -10                        # runs that reach position 0
+    base = (np.int64(9) << 34) if dtype == np.int64 else 0
+    s, e, qs, qe = ((x + base).astype(dtype) for x in (s, e, qs, qe))
+    ix = _dev_index(sia, s, e, np.arange(n, dtype=np.int32))
+    oix = orc.build_index(s, e)
+    ocount = orc.count_batch(oix, qs, qe)
+    st = oix["starts"]
+    run = np.searchsorted(st, qe, side="right") - np.searchsorted(st, qs, side="left")
+    assert run.sum() > 0.8 * ocount.sum()      # the run dominates
+    want = {o: orc.search_batch(oix, qs, qe, mode=m) for o, m in (("desc", 0), ("vector", 1))}
+    saved = autotune.current()
+    try:
+        for rs in (1, 0):
+            for ls in (1024, 2):
+                autotune.apply({"range_skip": rs, "lane_steps": ls, "count_rank": 0, "offsets_rank": 0})
+                assert np.array_equal(ix.count(qs, qe), ocount), (rs, ls)
+                for order in ("desc", "vector"):
+                    off, idx = ix.search(qs, qe, order=order)
+                    assert np.array_equal(off, want[order][0]) and np.array_equal(idx.astype(np.int64), want[order][1]), (rs, ls, order)
+                off, vals = ix.search(qs, qe, values=True)
+                assert np.array_equal(vals, oix["data"][want["desc"][1]])
+    finally:
+        autotune.apply(saved)
     ix.close()
 
 
