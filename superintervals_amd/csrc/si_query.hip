@@ -758,33 +758,174 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
     if (MODE != W_COUNT) emit_segments<S, MODE, EMIT_EVERY - 1>(held, lane, my_emit, s_kbit, out_idx, vals, out_val);
 }
 
-// Wave-cooperative continuation of parked walks: one wave per query, one aligned line of
-// 64 ends per step (a coalesced 256-byte load for int32), hits by ballot.  Output is
-// written at each hit's rank among the line's hits.  One round trip fetches the line
-// holding the walk position and the HEAVY_LINES - 1 lines below it, ends and exitp
-// together, into registers; the walk then steps through them (parked walks are long and
-// visit most lines of their span) and fetches again only when it leaves them.
+// Wave-cooperative walk of one query, guided by the max tree.  The hits at and below
+// position cur are {j <= cur : ends[j] >= s}, and an aligned 64-position line holds one iff
+// its maximum (level 1 of the max tree, one entry per line) is >= s.  The wave visits exactly
+// those lines, highest first, one 64-position line per step with hits by ballot: the lines of
+// the current group of 64 lines come from one ballot over their level-1 maxima (kept as a
+// wave-uniform mask), and when the group has none left, the nearest group to the left with a
+// line >= s is found through the levels above (prev_line, the search build uses for branch).
+// Up to WAVE_LINES such lines are fetched per round trip.  The walk reads no branch or exit
+// array: ends (or the 32-bit copy of int64 ends) and the small level-1 array only.
 //
-// Line exit.  With the trailing misses [b0, b0 + x] of the line (b0 not a hit), the walk
-// leaves through exitp[b0 + x], which is the exit of the 64-byte sub-line of exitp's own
-// width holding b0 + x.  A target still at or above b0 is a miss of a lower sub-line whose
-// entries below it are misses too, so its own exitp entry applies again: at most 64 / W
-// register lookups (readlane) give the first position below b0 that the reference walk
-// reaches, with no dependent memory access.
+// In the C++ vector order the leading run [i* + 1, top] is written ascending once its bottom
+// i* (the first miss below top) is found: the lead phase steps through consecutive lines
+// from cur while they are all hits.
 //
-// All 64 lanes must reach every ballot and shuffle together.  The values that steer the
-// loops (entry number, walk position) are therefore made wave-uniform with readfirstlane,
-// and the loop holds a single lane-dependent branch (lane 0's atomic).  With a second
-// lane-0-only block (a lane-0 store at the end of the body) the compiler merged the two
-// lane == 0 tests and sent lanes 1..63 back to the shuffle without lane 0; they read the
-// disabled lane, re-walked entry 0 and took an empty ballot's -1 as a line offset.
-#ifndef SI_HEAVY_LINES
-#define SI_HEAVY_LINES 4
+// All 64 lanes must reach every ballot together, so every value that steers a loop is
+// wave-uniform (ballot results, or readfirstlane).  (A kernel that held two lane == 0
+// blocks in one loop once had its tests merged by the compiler, which sent lanes 1..63 back
+// to a shuffle without lane 0; the loops here hold no lane-dependent branch.)
+#ifndef SI_WAVE_LINES
+#define SI_WAVE_LINES 4
 #endif
-constexpr uint32_t HEAVY_LINES = SI_HEAVY_LINES;   // 64-position lines a heavy walk fetches per round trip
+constexpr uint32_t WAVE_LINES = SI_WAVE_LINES;   // lines with hits fetched per round trip
 
+template <typename S>
+__device__ __forceinline__ S wave_end(const IndexView<S>& ix, uint32_t i) {
+    if (sizeof(S) == 8 && ix.ends32) return (S)((int64_t)ix.ends32[i] + ix.ends32_base);   // int64 ends within 2^32
+    return ix.ends[i];
+}
+
+// Highest line l < L0 whose level-1 maximum is >= s, or NONE: the entries [g, pos) of a
+// level's group by one ballot, else the level above over [0, g); once found, down to the
+// highest child >= s at each level (a parent's maximum is its children's).
+template <typename S>
+__device__ __forceinline__ uint32_t prev_line(const TreeDesc<S>& td, uint32_t L0, S s, int lane) {
+    constexpr S SMIN = (sizeof(S) == 4) ? (S)INT32_MIN : (S)INT64_MIN;
+    uint32_t pos = L0;   // exclusive bound at level l
+    for (int l = 1; l < td.nlev; ++l) {
+        if (pos == 0) return NONE;
+        const uint32_t g = (pos - 1u) & ~63u;
+        const uint32_t i = g + (uint32_t)lane;
+        const S v = (i < pos) ? td.lvl[l][i] : SMIN;
+        const uint64_t m = __ballot(v >= s);
+        if (m) {
+            uint32_t node = g + 63u - (uint32_t)__builtin_clzll(m);
+            for (int d = l - 1; d >= 1; --d) {
+                const uint32_t c = node * 64u + (uint32_t)lane;
+                const S w = (c < td.size[d]) ? td.lvl[d][c] : SMIN;
+                const uint64_t mm = __ballot(w >= s);
+                if (!mm) return NONE;   // cannot happen: the parent's maximum is a child's
+                node = node * 64u + 63u - (uint32_t)__builtin_clzll(mm);
+            }
+            return node;
+        }
+        pos = g / 64u;   // level l + 1 entries [0, g / 64) cover level l entries [0, g)
+    }
+    return NONE;
+}
+
+// Mask of the lines of group g1 (lines g1 .. g1 + 63) whose maximum is >= s.
+template <typename S>
+__device__ __forceinline__ uint64_t group_mask(const TreeDesc<S>& td, uint32_t g1, S s, int lane) {
+    const uint32_t i = g1 + (uint32_t)lane;
+    return __ballot(i < td.size[1] && td.lvl[1][i] >= s);
+}
+
+// Walks the hits at and below cur (see above).  acc: the count so far (count mode) or the
+// next output offset (fill modes, writes below oend only); left: in fill modes the hits
+// still expected, counted down.
 template <typename S, int MODE>
-__global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanView<S> pv,
+__device__ __forceinline__ void wave_walk(const IndexView<S>& ix, const TreeDesc<S>& td, int lane, S s, uint32_t cur,
+                                          uint32_t top, bool lead, uint64_t& acc, uint64_t oend, uint32_t& left,
+                                          uint32_t* __restrict__ out_idx, const int32_t* __restrict__ vals,
+                                          int32_t* __restrict__ out_val) {
+    const uint64_t above = (lane == 63) ? 0ull : (~0ull << (lane + 1));   // lanes above this one
+    // leading run (vector order): consecutive lines while every position down from cur is a hit
+    while (MODE == W_VECTOR && lead && cur != NONE) {
+        const uint32_t b0 = cur & ~63u;
+        const uint32_t kc = cur - b0;
+        const bool hit = ((uint32_t)lane <= kc) && (wave_end(ix, b0 + lane) >= s);
+        const uint64_t m = __ballot(hit);
+        const uint64_t lim = (kc == 63) ? ~0ull : ((2ull << kc) - 1ull);
+        const uint64_t inv = ~m & lim;
+        uint32_t lo = 0;
+        if (inv == 0) {
+            cur = b0 - 1u;   // b0 == 0 wraps to NONE
+        } else {
+            cur = b0 + 63u - (uint32_t)__builtin_clzll(inv);   // the first miss below the run
+            lo = cur + 1u;
+        }
+        if (inv != 0 || cur == NONE) {
+            left -= top - lo + 1u;
+            const uint64_t c = min((uint64_t)(top - lo + 1u), oend - acc);
+            for (uint64_t x = lane; x < c; x += WAVE) out_idx[acc + x] = lo + (uint32_t)x;
+            acc += c;
+            lead = false;
+        }
+    }
+    if (cur == NONE) return;
+    if (td.nlev < 2) {
+        // one line only (no level above the ends): its hits at and below cur
+        const bool hit = ((uint32_t)lane <= cur) && (wave_end(ix, (uint32_t)lane) >= s);
+        const uint64_t m = __ballot(hit);
+        const uint32_t nh = (uint32_t)__popcll(m);
+        if (MODE == W_COUNT) {
+            acc += nh;
+        } else {
+            left -= nh;
+            const uint64_t r = (uint64_t)__popcll(m & above);
+            if (hit && acc + r < oend) {
+                if (MODE == W_VALUES) out_val[acc + r] = vals[lane];
+                else out_idx[acc + r] = (uint32_t)lane;
+            }
+            acc = min(acc + nh, oend);
+        }
+        return;
+    }
+    const uint32_t L0 = cur >> 6;        // the first line, walked at and below cur only
+    uint32_t g1 = L0 & ~63u;             // group of 64 lines in hand
+    const uint32_t sh = L0 - g1;
+    uint64_t rem = group_mask(td, g1, s, lane) & ((sh == 63u) ? ~0ull : ((2ull << sh) - 1ull));
+    while (true) {
+        if (rem == 0) {
+            const uint32_t l = prev_line(td, g1, s, lane);
+            if (l == NONE) return;
+            g1 = l & ~63u;
+            rem = group_mask(td, g1, s, lane) & ((2ull << (l - g1)) - 1ull);   // lines of the group up to l
+            continue;
+        }
+        // the next WAVE_LINES lines with hits, highest first (NONE past the group's last)
+        uint32_t ln[WAVE_LINES];
+        S v[WAVE_LINES];
+#pragma unroll
+        for (uint32_t j = 0; j < WAVE_LINES; ++j) {
+            ln[j] = rem ? g1 + 63u - (uint32_t)__builtin_clzll(rem) : NONE;
+            rem = rem ? (rem & ~(1ull << (ln[j] - g1))) : 0ull;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < WAVE_LINES; ++j) {
+            const uint32_t b0 = (ln[j] == NONE) ? 0u : ln[j] * 64u;   // absent lines read line 0, unused
+            v[j] = wave_end(ix, b0 + (uint32_t)lane);
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < WAVE_LINES; ++j) {
+            if (ln[j] == NONE) continue;
+            const uint32_t b0 = ln[j] * 64u;
+            const uint32_t kc = (ln[j] == L0) ? cur - b0 : 63u;
+            const bool hit = ((uint32_t)lane <= kc) && (v[j] >= s);
+            const uint64_t m = __ballot(hit);
+            const uint32_t nh = (uint32_t)__popcll(m);
+            if (MODE == W_COUNT) {
+                acc += nh;
+            } else {
+                left -= nh;
+                const uint64_t r = (uint64_t)__popcll(m & above);   // hits above this lane
+                if (hit && acc + r < oend) {
+                    if (MODE == W_VALUES) out_val[acc + r] = vals[b0 + lane];
+                    else out_idx[acc + r] = b0 + (uint32_t)lane;
+                }
+                acc = min(acc + nh, oend);
+            }
+        }
+    }
+}
+
+// Continuation of walks the lane walk parked (over its step budget): one wave per parked
+// query, taken from a global counter.
+template <typename S, int MODE>
+__global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, TreeDesc<S> td, PlanView<S> pv,
                                                           uint32_t* __restrict__ counts,
                                                           uint64_t* __restrict__ counts64,
                                                           const uint64_t* __restrict__ offsets,
@@ -792,7 +933,6 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanV
                                                           const int32_t* __restrict__ vals,
                                                           int32_t* __restrict__ out_val, uint64_t cap) {
     const int lane = lane_id();
-    const uint64_t le_mask = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);   // lanes <= this
     const uint32_t nheavy = __builtin_amdgcn_readfirstlane(pv.counter[1]);
     if (nheavy == 0) return;   // nothing parked: no wave touches the shared counter
     while (true) {
@@ -804,86 +944,13 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, PlanV
         const uint32_t k = __builtin_amdgcn_readfirstlane(he.k);
         const uint32_t q = __builtin_amdgcn_readfirstlane(pv.q[k]);
         const S s = pv.s[k];
-        uint32_t cur = __builtin_amdgcn_readfirstlane(he.cur);
+        const uint32_t cur = __builtin_amdgcn_readfirstlane(he.cur);
         const uint32_t top = __builtin_amdgcn_readfirstlane(he.top);
-        bool lead = __builtin_amdgcn_readfirstlane(he.lead) != 0;
+        const bool lead = __builtin_amdgcn_readfirstlane(he.lead) != 0;
         uint32_t left = __builtin_amdgcn_readfirstlane(he.left);   // fill modes: hits still expected
         uint64_t acc = he.acc;
         const uint64_t oend = (MODE != W_COUNT) ? min(offsets[q + 1], cap) : 0;
-        while (cur != NONE) {
-            // fetch the line holding cur and the lines below it
-            const uint32_t tb = cur & ~63u;
-            S cv[HEAVY_LINES];
-            uint32_t cx[HEAVY_LINES];
-#pragma unroll
-            for (uint32_t l = 0; l < HEAVY_LINES; ++l) {
-                // lines below position 0 are never stepped into; line 0 stands in for them
-                const uint32_t lb = (tb >= 64u * l) ? tb - 64u * l : 0u;
-                if (sizeof(S) == 8 && ix.ends32)   // int64 ends within 2^32 of their minimum
-                    cv[l] = (S)((int64_t)ix.ends32[lb + lane] + ix.ends32_base);
-                else
-                    cv[l] = ix.ends[lb + lane];
-                cx[l] = ix.exitp[lb + lane];
-            }
-            // the walk leaves every line it steps into and only moves down, so each fetched
-            // line is stepped into at most once, in this order (a line below position 0
-            // has a wrapped base that no position matches)
-#pragma unroll
-            for (uint32_t l = 0; l < HEAVY_LINES; ++l) {
-                const uint32_t b0 = tb - 64u * l;
-                if (cur == NONE || (cur & ~63u) != b0) continue;
-                const uint32_t kc = cur - b0;
-                const S v = cv[l];
-                const uint32_t ex = cx[l];
-                const bool hit = ((uint32_t)lane <= kc) && (v >= s);
-                const uint64_t m = __ballot(hit);
-                if (MODE == W_VECTOR && lead) {
-                    const uint64_t lim = (kc == 63) ? ~0ull : ((2ull << kc) - 1ull);
-                    const uint64_t inv = ~m & lim;
-                    uint32_t lo;
-                    if (inv == 0) {
-                        cur = b0 - 1u;
-                        lo = 0;
-                    } else {
-                        const uint32_t hm = 63u - (uint32_t)__builtin_clzll(inv);
-                        cur = b0 + hm;   // the first miss below the run: this line again
-                        lo = cur + 1u;
-                    }
-                    if (inv != 0 || cur == NONE) {
-                        left -= top - lo + 1u;
-                        const uint64_t c = min((uint64_t)(top - lo + 1u), oend - acc);
-                        for (uint64_t x = lane; x < c; x += WAVE) out_idx[acc + x] = lo + (uint32_t)x;
-                        acc += c;
-                        lead = false;
-                    }
-                    if (inv != 0) break;   // the walk resumes inside this line: fetch again
-                    continue;
-                }
-                const uint32_t nh = (uint32_t)__popcll(m);
-                if (MODE == W_COUNT) {
-                    acc += nh;
-                } else {
-                    left -= nh;
-                    if (hit) {
-                        const uint64_t r = (uint64_t)__popcll(m & ~le_mask);   // hits above this lane
-                        if (acc + r < oend) {
-                            if (MODE == W_VALUES) out_val[acc + r] = vals[b0 + lane];
-                            else out_idx[acc + r] = b0 + (uint32_t)lane;
-                        }
-                    }
-                    acc = min(acc + nh, oend);
-                }
-                if (m & 1ull) {
-                    cur = b0 - 1u;
-                } else {
-                    const uint32_t x = m ? (uint32_t)(__builtin_ctzll(m) - 1) : kc;
-                    uint32_t p = (uint32_t)__builtin_amdgcn_readlane((int)ex, (int)x);
-                    // each lookup moves to a lower sub-line of this line, or leaves it
-                    while (p != NONE && p >= b0) p = (uint32_t)__builtin_amdgcn_readlane((int)ex, (int)(p - b0));
-                    cur = p;
-                }
-            }
-        }
+        wave_walk<S, MODE>(ix, td, lane, s, cur, top, lead, acc, oend, left, out_idx, vals, out_val);
         if (MODE == W_COUNT) {
             // every lane stores the same value (no lane-dependent branch in the loop)
             if (counts) counts[q] = (uint32_t)acc;
@@ -1224,8 +1291,8 @@ int launch_walk(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint
     }
     IndexView<S> hv = ix->view<S>();
     if (!g_tune.heavy_ends32) hv.ends32 = nullptr;
-    k_walk_heavy<S, MODE><<<(unsigned)cus * 8u, Q_THREADS, 0, st>>>(hv, pv, counts, counts64, offsets,
-                                                                   out_idx, ix->d_values, out_val, cap);
+    k_walk_heavy<S, MODE><<<(unsigned)cus * 8u, Q_THREADS, 0, st>>>(hv, ix->tree<S>(), pv, counts, counts64,
+                                                                   offsets, out_idx, ix->d_values, out_val, cap);
     LAUNCH_CHECK();
     return SI_OK;
 }
