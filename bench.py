@@ -134,6 +134,56 @@ def bench_learned(a):
         print("[learned] %s: %.3f ms (plain %.3f ms, tree %.3f ms), identical %s, max error %d" %
               (name, t, t_plain, t_tree, same, info["max_abs_err"]), file=sys.stderr, flush=True)
         m.close()
+    # the batched search (plan -> offsets -> fill, descending) with the plan's bounds found by
+    # plain halving, by the tree and by the trained model: identical output, timed per stage
+    m = sia.LearnedIndex(ix, 4096)
+    m.train(steps=a.learned_steps, batch=65536, lr=3e-3, stream=stream)
+    qs, qe = sia.workload_queries(4, nq, seed=42)
+    tqs, tqe = torch.from_numpy(qs).to(dev), torch.from_numpy(qe).to(dev)
+    del qs, qe
+    plan = torch.empty(ix.plan_bytes(nq), dtype=torch.uint8, device=dev)
+    off = torch.empty(nq + 1, dtype=torch.int64, device=dev)
+    ix.plan_device(tqs, tqe, plan, stream=stream)
+    ix.offsets_planned(plan, nq, off, stream=stream)
+    torch.cuda.synchronize()
+    total = int(off[-1].item())
+    idx = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+    ref_idx = ref_off = None
+    search = {}
+    saved = autotune.current()
+    try:
+        for name, knobs in (("halving", {"ub_tree": 0, "ub_learned": 0}), ("tree", {"ub_tree": 1, "ub_learned": 0}),
+                            ("learned", {"ub_tree": 1, "ub_learned": 1})):
+            autotune.apply(dict(knobs, plan_sync=0))
+            ts = {"plan": 0.0, "offsets": 0.0, "fill": 0.0}
+            for rep in range(a.steps + 1):
+                ev = events(4)
+                ev[0].record(stream)
+                ix.plan_device(tqs, tqe, plan, stream=stream)
+                ev[1].record(stream)
+                ix.offsets_planned(plan, nq, off, stream=stream)
+                ev[2].record(stream)
+                ix.fill_planned(plan, nq, off, idx, order="desc", stream=stream)
+                ev[3].record(stream)
+                torch.cuda.synchronize()
+                if rep:                                  # the first pass is a warm-up
+                    ts["plan"] += ev[0].elapsed_time(ev[1]) / a.steps
+                    ts["offsets"] += ev[1].elapsed_time(ev[2]) / a.steps
+                    ts["fill"] += ev[2].elapsed_time(ev[3]) / a.steps
+            if ref_idx is None:
+                ref_off, ref_idx = off.clone(), idx.clone()
+                same = True
+            else:
+                same = bool(torch.equal(off, ref_off)) and bool(torch.equal(idx, ref_idx))
+            step = ts["plan"] + ts["offsets"] + ts["fill"]
+            search[name] = dict({k: round(v, 4) for k, v in ts.items()}, step_ms=round(step, 4),
+                                queries_per_s=nq / (step / 1e3), identical_to_halving=same)
+            print("[learned] batched search, bounds by %s: plan %.3f ms, offsets %.3f ms, fill %.3f ms, identical %s"
+                  % (name, ts["plan"], ts["offsets"], ts["fill"], same), file=sys.stderr, flush=True)
+    finally:
+        autotune.apply(saved)
+    m.close()
+    del plan, idx, ref_idx
     tr = res["trained"]
     out = {
         "metric": "learned upper_bound queries/s vs plain binary search (200M idx x 50M point/short qry)",
@@ -149,6 +199,9 @@ def bench_learned(a):
         "speedup_over_plain": t_plain / tr["ms"],
         "speedup_over_tree": t_tree / tr["ms"],
         "max_abs_err": tr["max_abs_err"],
+        "batched_search": dict(search, hits=total,
+                               learned_over_halving=search["halving"]["step_ms"] / search["learned"]["step_ms"],
+                               learned_over_tree=search["tree"]["step_ms"] / search["learned"]["step_ms"]),
     }
     print(json.dumps(out))
 

@@ -134,6 +134,15 @@ double siIndexLastBuildMs(const siIndex* ix);
  *   "plan_skip_low" 0 (default) .. 2: lowest 8-bit digits of the query end that
  *                  siPlanQueries leaves unsorted (one radix pass fewer each; the walk
  *                  allows the coarser order at some cost in LDS window hits)
+ *   "range_skip"   1 (default): the plan also finds each query's lower_bound and the walks
+ *                  take [lower_bound(start), upper_bound(end)] as hits without reading ends
+ *                  (exact unless the index holds an interval with start > end, where it is
+ *                  off); 0: they walk that run too
+ *   "ub_learned"   0 (default); 1: siPlanQueries finds upper_bound / lower_bound through the
+ *                  learned model trained on the index (siLearnedTrain attaches it), when the
+ *                  index has one; results are identical either way
+ *   "learned_shrink" 0 (default); a test knob: the learned windows are narrowed by this many
+ *                  positions per side, forcing the model's whole-leaf fallback
  * superintervals_amd/autotune.py measures candidates and stores the best per workload. */
 int siTuneSet(const char* name, int64_t value);
 int64_t siTuneGet(const char* name);   /* -1 for an unknown name */
@@ -308,7 +317,9 @@ void siLearnedDestroy(siLearned* m);
 /* Train on the GPU: `steps` Adam steps of minibatch MSE (batch samples of the index's own
  * (start, position / n) pairs, drawn deterministically), then fit the leaves and their
  * error bounds.  steps = 0 keeps the identity-CDF initialisation (routing by key alone).
- * Synchronises the stream. */
+ * Attaches the model to its index, so that with the tuning knob "ub_learned" = 1 the plan
+ * of every batch (siPlanQueries, and the unplanned calls) finds its bounds through it;
+ * siLearnedDestroy and rebuilding the index detach it.  Synchronises the stream. */
 int siLearnedTrain(siLearned* m, int steps, uint32_t batch, float lr, void* stream);
 
 /* Number of root parameters (W1[16], b1[16], W2[16][16], b2[16], w3[16], b3). */

@@ -24,3 +24,14 @@ int si_intersect_host(const siIndex* ix, const int32_t* as, const int32_t* ae, s
                       std::vector<int32_t>& pe);
 int si_difference_host(const siIndex* ix, const int32_t* as, const int32_t* ae, const int32_t* ad, size_t na,
                        std::vector<int32_t>& ps, std::vector<int32_t>& pe, std::vector<int32_t>& pd);
+
+// Learned position model (si_learned.hip) for the plan: idx[k] = upper_bound(qe[k]) and, with
+// lb, lb[k] = the first position whose start is >= qs[k], through the model attached to ix.
+// SI_ERR_ARG when ix has no trained model of its current build.
+int si_learned_plan_bounds(const siIndex* ix, const void* qe, const void* qs, size_t nq, uint32_t* idx,
+                           uint32_t* lb, void* stream);
+// Test-only tuning: every learned leaf window narrowed by this many positions on each side
+// (forces the whole-leaf fallback); 0 in normal use.
+int si_tune_learned_shrink();
+// The index a model was created on is being destroyed: the model forgets it.
+void si_learned_forget_index(siLearned* m);

@@ -498,6 +498,7 @@ SI_EXPORT siIndex* siIndexCreate(int coord_bytes) {
 
 SI_EXPORT void siIndexDestroy(siIndex* ix) {
     if (!ix) return;
+    for (siLearned* m : ix->models) si_learned_forget_index(m);
     ix->release();
     delete ix;
 }

@@ -3,6 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "si_common.h"
 #include "si_kernels.h"
 
@@ -49,6 +52,11 @@ struct siIndex {
     // [first start >= s, upper_bound] both assume start <= end, so queries on an index
     // holding any such interval take the walk
     size_t n_inverted = 0;
+    // the learned position model trained on this build (siLearnedTrain attaches it,
+    // siLearnedDestroy and the next build detach it); generation counts builds
+    siLearned* learned = nullptr;
+    uint64_t generation = 0;
+    std::vector<siLearned*> models;   // every model created on this index (siIndexDestroy unlinks them)
     KeyTree ub;   // search tree over the starts (upper_bound)
     KeyTree eb;   // search tree over the sorted ends (rank count)
     int nlev = 0;
@@ -63,6 +71,8 @@ struct siIndex {
         d_ends32 = nullptr;
         ends32_base = 0;
         n_inverted = 0;
+        learned = nullptr;
+        ++generation;
         ub.release();
         eb.release();
         d_branch = d_exit = d_perm = nullptr;

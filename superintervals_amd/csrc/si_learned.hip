@@ -320,11 +320,97 @@ __device__ __forceinline__ int64_t last_le(const S* __restrict__ st, int64_t lo,
     return (int64_t)(base - st) - ((*base <= x) ? 0 : 1);
 }
 
-// Learned upper_bound.  The root names a leaf; a galloping search of the leaves' first
-// keys in LDS corrects it to the leaf holding upper_bound(x); the leaf's window (which
-// provably contains the answer) is searched; a one-read check guards the result and falls
-// back to a search of the whole leaf.  Persistent blocks stage the root and the first keys
-// once.  stats (optional): [0] fallbacks, [1] window widths, [2] leaf corrections.
+// The leaf holding upper_bound(x): from the guess lg, gallop through the leaves' first keys
+// (LDS) to the bracket s_kb[lo] <= x < s_kb[hi], then halve it.  -1 when x is below every
+// first key.
+template <typename S>
+__device__ __forceinline__ int32_t correct_leaf(const S* __restrict__ s_kb, uint32_t L, int32_t lg, S x) {
+    int32_t lo, hi;
+    if (s_kb[lg] <= x) {
+        lo = lg;
+        hi = lg + 1;
+        int32_t step = 1;
+        while (hi < (int32_t)L && s_kb[hi] <= x) {
+            lo = hi;
+            step <<= 1;
+            hi = lo + step;
+        }
+        hi = std::min(hi, (int32_t)L);
+    } else {
+        hi = lg;
+        lo = lg - 1;
+        int32_t step = 1;
+        while (lo >= 0 && s_kb[lo] > x) {
+            hi = lo;
+            step <<= 1;
+            lo = hi - step;
+        }
+        lo = std::max(lo, -1);
+    }
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (s_kb[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// upper_bound(x) inside leaf lo (whose first key is <= x): the leaf's prediction window
+// (narrowed by shrink per side, a test of the fallback), a binary search of starts in it,
+// a one-read check that the result is the last start <= x, and the whole leaf as fallback.
+template <typename S>
+__device__ __forceinline__ uint32_t leaf_search(const IndexView<S>& ix, const Leaf* __restrict__ leaves, int32_t lo,
+                                                S x, int32_t shrink, unsigned long long& f_acc,
+                                                unsigned long long& w_acc) {
+    if (lo < 0) return NONE;
+    const Leaf lf = leaves[lo];
+    const int64_t pl = lf.base, ph = lf.last;
+    const int64_t p = leaf_pred_c(lf, (int64_t)x, pl, ph);
+    const int64_t wl = std::max<int64_t>(p + lf.elo + shrink, pl);
+    const int64_t wh = std::min<int64_t>(p + lf.ehi - shrink, ph);
+    bool ok = false;
+    int64_t r = pl - 1;
+    if (wl <= wh) {
+        w_acc += (unsigned long long)(wh - wl + 1);
+        r = last_le(ix.starts, wl, wh, x);
+        ok = (r >= wl) && (r < wh || wh == ph || ix.starts[wh + 1] > x);
+    }
+    if (!ok) {
+        f_acc += 1;
+        r = last_le(ix.starts, pl, ph, x);   // starts[pl] <= x, so r >= pl
+    }
+    return (uint32_t)r;
+}
+
+// Persistent blocks stage the root's parameters and the leaves' first keys in LDS.
+template <typename S>
+__device__ __forceinline__ void stage_model(float* s_p, S* s_kb, const float* __restrict__ P, const S* __restrict__ kb,
+                                            uint32_t L) {
+    for (int k = threadIdx.x; k < NPARAM; k += RMI_THREADS) s_p[k] = P[k];
+    for (uint32_t k = threadIdx.x; k < L; k += RMI_THREADS) s_kb[k] = kb[k];
+    __syncthreads();
+}
+
+template <typename S>
+__device__ __forceinline__ void add_stats(unsigned long long* stats, unsigned long long f_acc,
+                                          unsigned long long w_acc, unsigned long long c_acc) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        f_acc += __shfl_xor(f_acc, d, WAVE);
+        w_acc += __shfl_xor(w_acc, d, WAVE);
+        c_acc += __shfl_xor(c_acc, d, WAVE);
+    }
+    if (lane_id() == 0) {
+        atomicAdd(&stats[0], f_acc);
+        atomicAdd(&stats[1], w_acc);
+        atomicAdd(&stats[2], c_acc);
+    }
+}
+
+// Learned upper_bound.  The root names a leaf; the first keys correct it to the leaf holding
+// upper_bound(x); the leaf's window, which provably contains the answer, is searched, with a
+// one-read check and the whole leaf as fallback.  stats (optional): [0] fallbacks, [1] window
+// widths, [2] leaf corrections.
 template <typename S>
 __global__ __launch_bounds__(RMI_THREADS) void k_ub_learned(IndexView<S> ix, int64_t kmin, float inv_span,
                                                             const float* __restrict__ P,
@@ -332,80 +418,60 @@ __global__ __launch_bounds__(RMI_THREADS) void k_ub_learned(IndexView<S> ix, int
                                                             const S* __restrict__ kb, uint32_t L,
                                                             const S* __restrict__ xs, size_t nx,
                                                             uint32_t* __restrict__ out,
-                                                            unsigned long long* __restrict__ stats) {
+                                                            unsigned long long* __restrict__ stats, int32_t shrink) {
     extern __shared__ __align__(16) char s_mem[];
     float* s_p = reinterpret_cast<float*>(s_mem);
     S* s_kb = reinterpret_cast<S*>(s_mem + ((NPARAM * 4 + 15) & ~15));
-    for (int k = threadIdx.x; k < NPARAM; k += RMI_THREADS) s_p[k] = P[k];
-    for (uint32_t k = threadIdx.x; k < L; k += RMI_THREADS) s_kb[k] = kb[k];
-    __syncthreads();
+    stage_model(s_p, s_kb, P, kb, L);
     unsigned long long f_acc = 0, w_acc = 0, c_acc = 0;
     for (size_t q0 = (size_t)blockIdx.x * RMI_THREADS; q0 < nx; q0 += (size_t)gridDim.x * RMI_THREADS) {
         const size_t q = q0 + threadIdx.x;
         if (q >= nx) continue;
         const S x = xs[q];
         const int32_t lg = (int32_t)leaf_of(root_forward(s_p, norm_key(x, kmin, inv_span)), L);
-        // bracket the leaf: s_kb[lo] <= x < s_kb[hi], lo in [-1, L - 1], hi in [lo + 1, L]
-        int32_t lo, hi;
-        if (s_kb[lg] <= x) {
-            lo = lg;
-            hi = lg + 1;
-            int32_t step = 1;
-            while (hi < (int32_t)L && s_kb[hi] <= x) {
-                lo = hi;
-                step <<= 1;
-                hi = lo + step;
-            }
-            hi = std::min(hi, (int32_t)L);
-        } else {
-            hi = lg;
-            lo = lg - 1;
-            int32_t step = 1;
-            while (lo >= 0 && s_kb[lo] > x) {
-                hi = lo;
-                step <<= 1;
-                lo = hi - step;
-            }
-            lo = std::max(lo, -1);
-        }
-        while (hi - lo > 1) {
-            const int32_t mid = (lo + hi) >> 1;
-            if (s_kb[mid] <= x) lo = mid;
-            else hi = mid;
-        }
-        uint32_t res = NONE;
-        if (lo >= 0) {
-            c_acc += (unsigned long long)(lo > lg ? lo - lg : lg - lo);
-            const Leaf lf = leaves[lo];
-            const int64_t pl = lf.base, ph = lf.last;
-            const int64_t p = leaf_pred_c(lf, (int64_t)x, pl, ph);
-            const int64_t wl = std::max<int64_t>(p + lf.elo, pl), wh = std::min<int64_t>(p + lf.ehi, ph);
-            bool ok = false;
-            int64_t r = pl - 1;
-            if (wl <= wh) {
-                w_acc += (unsigned long long)(wh - wl + 1);
-                r = last_le(ix.starts, wl, wh, x);
-                ok = (r >= wl) && (r < wh || wh == ph || ix.starts[wh + 1] > x);
-            }
-            if (!ok) {
-                f_acc += 1;
-                r = last_le(ix.starts, pl, ph, x);   // starts[pl] <= x, so r >= pl
-            }
-            res = (uint32_t)r;
-        }
-        out[q] = res;
+        const int32_t lo = correct_leaf(s_kb, L, lg, x);
+        if (lo >= 0) c_acc += (unsigned long long)(lo > lg ? lo - lg : lg - lo);
+        out[q] = leaf_search(ix, leaves, lo, x, shrink, f_acc, w_acc);
     }
-    if (stats) {
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            f_acc += __shfl_xor(f_acc, d, WAVE);
-            w_acc += __shfl_xor(w_acc, d, WAVE);
-            c_acc += __shfl_xor(c_acc, d, WAVE);
-        }
-        if (lane_id() == 0) {
-            atomicAdd(&stats[0], f_acc);
-            atomicAdd(&stats[1], w_acc);
-            atomicAdd(&stats[2], c_acc);
+    if (stats) add_stats<S>(stats, f_acc, w_acc, c_acc);
+}
+
+// The plan's two bounds through the model with one root evaluation per query: idx[k] =
+// upper_bound(e[k]) from the root's leaf, and lb[k] = the first position starting at or after
+// s[k] = upper_bound(s[k] - 1) + 1 (0 when s[k] is the smallest value), its leaf corrected
+// from e's leaf (the query's start is close below its end).  The leaf correction is exact
+// from any guess, so both results equal the plain search's.
+template <typename S>
+__global__ __launch_bounds__(RMI_THREADS) void k_plan_bounds_learned(IndexView<S> ix, int64_t kmin, float inv_span,
+                                                                     const float* __restrict__ P,
+                                                                     const Leaf* __restrict__ leaves,
+                                                                     const S* __restrict__ kb, uint32_t L,
+                                                                     const S* __restrict__ qe,
+                                                                     const S* __restrict__ qs, size_t nq,
+                                                                     uint32_t* __restrict__ idx,
+                                                                     uint32_t* __restrict__ lb, int32_t shrink) {
+    constexpr S SMIN = (sizeof(S) == 4) ? (S)INT32_MIN : (S)INT64_MIN;
+    extern __shared__ __align__(16) char s_mem[];
+    float* s_p = reinterpret_cast<float*>(s_mem);
+    S* s_kb = reinterpret_cast<S*>(s_mem + ((NPARAM * 4 + 15) & ~15));
+    stage_model(s_p, s_kb, P, kb, L);
+    unsigned long long f_acc = 0, w_acc = 0;
+    for (size_t q0 = (size_t)blockIdx.x * RMI_THREADS; q0 < nq; q0 += (size_t)gridDim.x * RMI_THREADS) {
+        const size_t q = q0 + threadIdx.x;
+        if (q >= nq) continue;
+        const S x = qe[q];
+        const int32_t lg = (int32_t)leaf_of(root_forward(s_p, norm_key(x, kmin, inv_span)), L);
+        const int32_t lo = correct_leaf(s_kb, L, lg, x);
+        idx[q] = leaf_search(ix, leaves, lo, x, shrink, f_acc, w_acc);
+        if (lb) {
+            const S s = qs[q];
+            if (s == SMIN) {
+                lb[q] = 0u;
+            } else {
+                const S x2 = (S)(s - 1);
+                const int32_t lo2 = correct_leaf(s_kb, L, lo < 0 ? 0 : lo, x2);
+                lb[q] = leaf_search(ix, leaves, lo2, x2, shrink, f_acc, w_acc) + 1u;   // NONE + 1 wraps to 0
+            }
         }
     }
 }
@@ -417,6 +483,7 @@ __global__ __launch_bounds__(RMI_THREADS) void k_ub_learned(IndexView<S> ix, int
 // ============================================================================
 struct siLearned {
     uint32_t L = 4096;
+    uint64_t generation = 0;   // the index build it was trained on
     int64_t kmin = 0;
     float inv_span = 1.f;
     float* d_params = nullptr;   // NPARAM
@@ -526,18 +593,32 @@ SI_EXPORT siLearned* siLearnedCreate(const siIndex* ix, uint32_t leaves) {
     siLearned* m = new siLearned();
     m->L = (uint32_t)std::min<size_t>(leaves, ix->n);   // every leaf holds at least one position
     m->ix = ix;
+    const_cast<siIndex*>(ix)->models.push_back(m);
     m->ix_n = ix->n;
+    m->generation = ix->generation;
     if (hipMalloc(&m->d_params, NPARAM * 4) != hipSuccess ||
         hipMalloc(&m->d_leaves, (size_t)m->L * sizeof(Leaf)) != hipSuccess ||
         hipMalloc(&m->d_kb, (size_t)m->L * ix->cb) != hipSuccess) {
         si_set_error(SI_ERR_NOMEM, "learned index allocation");
-        delete m;
+        siLearnedDestroy(m);
         return nullptr;
     }
     return m;
 }
 
-SI_EXPORT void siLearnedDestroy(siLearned* m) { delete m; }
+SI_EXPORT void siLearnedDestroy(siLearned* m) {
+    if (!m) return;
+    if (m->ix) {   // the index still exists: unlink the model from it
+        siIndex* ix = const_cast<siIndex*>(m->ix);
+        if (ix->learned == m) ix->learned = nullptr;
+        ix->models.erase(std::remove(ix->models.begin(), ix->models.end(), m), ix->models.end());
+    }
+    delete m;
+}
+
+void si_learned_forget_index(siLearned* m) {
+    if (m) m->ix = nullptr;
+}
 
 SI_EXPORT int siLearnedParamCount(void) { return NPARAM; }
 
@@ -547,11 +628,14 @@ SI_EXPORT int siLearnedTrain(siLearned* m, int steps, uint32_t batch, float lr, 
     const siIndex* ix = m->ix;
     int rc = check_ix(ix);
     if (rc) return rc;
-    if (ix->n != m->ix_n) return si_set_error(SI_ERR_ARG, "the index was rebuilt; create a new model");
+    if (ix->generation != m->generation) return si_set_error(SI_ERR_ARG, "the index was rebuilt; create a new model");
     hipStream_t st = (hipStream_t)stream;
     rc = (ix->cb == 4) ? train_impl<int32_t>(m, ix, steps, batch, lr, st)
                        : train_impl<int64_t>(m, ix, steps, batch, lr, st);
-    if (rc == SI_OK) m->trained = true;
+    if (rc == SI_OK) {
+        m->trained = true;
+        const_cast<siIndex*>(ix)->learned = m;   // the plan may use it (tuning knob ub_learned)
+    }
     return rc;
 }
 
@@ -622,6 +706,12 @@ SI_EXPORT int siLearnedNorm(const siLearned* m, int64_t* kmin, float* inv_span) 
     return SI_OK;
 }
 
+namespace {
+unsigned learned_grid(size_t n);
+int launch_learned(const siLearned* m, const siIndex* ix, const void* xs, size_t n, uint32_t* out,
+                   unsigned long long* stats, hipStream_t st);
+}  // namespace
+
 // upper_bound through the model (device buffers).  stats (device, may be NULL) receives
 // [0] += fallbacks to the whole-leaf search, [1] += sum of window widths, [2] += sum of
 // |leaf - root's leaf| corrections.
@@ -631,10 +721,37 @@ SI_EXPORT int siUpperBoundLearned(const siLearned* m, const void* xs, size_t n, 
     const siIndex* ix = m->ix;
     int rc = check_ix(ix);
     if (rc) return rc;
-    if (ix->n != m->ix_n) return si_set_error(SI_ERR_ARG, "the index was rebuilt; retrain the model");
+    if (ix->generation != m->generation) return si_set_error(SI_ERR_ARG, "the index was rebuilt; retrain the model");
     if (n == 0) return SI_OK;
     if (!xs || !out) return si_set_error(SI_ERR_ARG, "null buffer");
+    return launch_learned(m, ix, xs, n, out, (unsigned long long*)stats, (hipStream_t)stream);
+}
+
+int si_learned_plan_bounds(const siIndex* ix, const void* qe, const void* qs, size_t nq, uint32_t* idx,
+                           uint32_t* lb, void* stream) {
+    const siLearned* m = ix ? ix->learned : nullptr;
+    if (!m || !m->trained || m->generation != ix->generation)
+        return si_set_error(SI_ERR_ARG, "ub_learned: the index has no trained learned model of its current build");
+    if (nq == 0) return SI_OK;
     hipStream_t st = (hipStream_t)stream;
+    const unsigned g = learned_grid(nq);
+    const size_t lds = ((NPARAM * 4 + 15) & ~15) + (size_t)m->L * ix->cb;
+    const int32_t shrink = (int32_t)si_tune_learned_shrink();
+    if (ix->cb == 4)
+        k_plan_bounds_learned<int32_t><<<g, RMI_THREADS, lds, st>>>(
+            ix->view<int32_t>(), m->kmin, m->inv_span, m->d_params, m->d_leaves, (const int32_t*)m->d_kb, m->L,
+            (const int32_t*)qe, (const int32_t*)qs, nq, idx, lb, shrink);
+    else
+        k_plan_bounds_learned<int64_t><<<g, RMI_THREADS, lds, st>>>(
+            ix->view<int64_t>(), m->kmin, m->inv_span, m->d_params, m->d_leaves, (const int64_t*)m->d_kb, m->L,
+            (const int64_t*)qe, (const int64_t*)qs, nq, idx, lb, shrink);
+    LAUNCH_CHECK();
+    return SI_OK;
+}
+
+namespace {
+// Persistent grid: enough blocks to fill the GPU, each stages the model once.
+unsigned learned_grid(size_t n) {
     static int cus = 0;
     if (cus == 0) {
         int dev = 0;
@@ -642,16 +759,23 @@ SI_EXPORT int siUpperBoundLearned(const siLearned* m, const void* xs, size_t n, 
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         cus = std::max(cus, 1);
     }
-    const unsigned g = (unsigned)std::min<size_t>(grid_for(n, RMI_THREADS), (size_t)cus * 8);
+    return (unsigned)std::min<size_t>(grid_for(n, RMI_THREADS), (size_t)cus * 8);
+}
+
+int launch_learned(const siLearned* m, const siIndex* ix, const void* xs, size_t n, uint32_t* out,
+                   unsigned long long* stats, hipStream_t st) {
+    const unsigned g = learned_grid(n);
     const size_t lds = ((NPARAM * 4 + 15) & ~15) + (size_t)m->L * ix->cb;
+    const int32_t shrink = (int32_t)si_tune_learned_shrink();
     if (ix->cb == 4)
         k_ub_learned<int32_t><<<g, RMI_THREADS, lds, st>>>(ix->view<int32_t>(), m->kmin, m->inv_span, m->d_params,
                                                            m->d_leaves, (const int32_t*)m->d_kb, m->L,
-                                                           (const int32_t*)xs, n, out, (unsigned long long*)stats);
+                                                           (const int32_t*)xs, n, out, stats, shrink);
     else
         k_ub_learned<int64_t><<<g, RMI_THREADS, lds, st>>>(ix->view<int64_t>(), m->kmin, m->inv_span, m->d_params,
                                                            m->d_leaves, (const int64_t*)m->d_kb, m->L,
-                                                           (const int64_t*)xs, n, out, (unsigned long long*)stats);
+                                                           (const int64_t*)xs, n, out, stats, shrink);
     LAUNCH_CHECK();
     return SI_OK;
 }
+}  // namespace

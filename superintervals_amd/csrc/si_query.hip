@@ -247,6 +247,10 @@ __global__ __launch_bounds__(Q_THREADS) void k_count_rank(IndexView<S> ix, const
 //   range_skip   1: the plan also finds each query's lower_bound and the walks take the
 //                run [lower_bound, upper_bound] (every position there starts inside the
 //                query) as hits without reading ends; 0: they walk it
+//   ub_learned   1: the plan's upper_bound / lower_bound go through the learned model
+//                trained on the index (siLearnedTrain), when it has one; 0: the tree
+//   learned_shrink  test only: the learned model's leaf windows narrowed by this many
+//                positions per side, which forces its whole-leaf fallback (0 normally)
 struct Tuning {
     uint32_t tile = 1024;
     int count_wgs = 0;
@@ -259,6 +263,8 @@ struct Tuning {
     int plan_skip_low = 0;
     int heavy_ends32 = 1;
     int range_skip = 1;
+    int ub_learned = 0;
+    int learned_shrink = 0;
 };
 Tuning g_tune;
 
@@ -291,11 +297,13 @@ int plan_impl(const siIndex* ix, const S* qs, const S* qe, size_t nq, void* plan
     if (rc != SI_OK) return rc;
     // upper_bound of the sorted ends (and lower_bound of the starts when the walks take the
     // guaranteed run): through the search tree when the index has one
-    const bool tree = g_tune.ub_tree && ix->ub.d_top && ix->n > 0;
+    const bool learned = g_tune.ub_learned && ix->learned;
+    const bool tree = !learned && g_tune.ub_tree && ix->ub.d_top && ix->n > 0;
     const bool with_lb = range_skip_on(ix);
     const PlanView<S> pv = plan_view<S>(plan, nq);
-    k_plan_finish<S><<<g, Q_THREADS, 0, st>>>(ix->view<S>(), ks, vs, qs, nq, pv, !tree, with_lb);
+    k_plan_finish<S><<<g, Q_THREADS, 0, st>>>(ix->view<S>(), ks, vs, qs, nq, pv, !tree && !learned, with_lb);
     LAUNCH_CHECK();
+    if (learned) return si_learned_plan_bounds(ix, pv.e, pv.s, nq, pv.idx, with_lb ? pv.lb : nullptr, st);
     if (tree) return launch_plan_bounds<S>(ix, pv, nq, with_lb, st);
     return SI_OK;
 }
@@ -1327,6 +1335,8 @@ struct ScopedPlan {
 // ============================================================================
 // Tuning
 // ============================================================================
+int si_tune_learned_shrink() { return g_tune.learned_shrink; }
+
 SI_EXPORT int siTuneSet(const char* name, int64_t value) {
     if (!name) return si_set_error(SI_ERR_ARG, "null tuning name");
     const std::string k(name);
@@ -1351,6 +1361,12 @@ SI_EXPORT int siTuneSet(const char* name, int64_t value) {
     } else if (k == "heavy_ends32") {
         if (value != 0 && value != 1) return si_set_error(SI_ERR_ARG, "heavy_ends32 must be 0 or 1");
         g_tune.heavy_ends32 = (int)value;
+    } else if (k == "ub_learned") {
+        if (value != 0 && value != 1) return si_set_error(SI_ERR_ARG, "ub_learned must be 0 or 1");
+        g_tune.ub_learned = (int)value;
+    } else if (k == "learned_shrink") {
+        if (value < 0 || value > (1 << 20)) return si_set_error(SI_ERR_ARG, "learned_shrink must be in [0, 2^20]");
+        g_tune.learned_shrink = (int)value;
     } else if (k == "range_skip") {
         if (value != 0 && value != 1) return si_set_error(SI_ERR_ARG, "range_skip must be 0 or 1");
         g_tune.range_skip = (int)value;
@@ -1378,6 +1394,8 @@ SI_EXPORT int64_t siTuneGet(const char* name) {
     if (k == "plan_skip_low") return g_tune.plan_skip_low;
     if (k == "heavy_ends32") return g_tune.heavy_ends32;
     if (k == "range_skip") return g_tune.range_skip;
+    if (k == "ub_learned") return g_tune.ub_learned;
+    if (k == "learned_shrink") return g_tune.learned_shrink;
     if (k == "count_rank") return g_tune.count_rank;
     if (k == "offsets_rank") return g_tune.offsets_rank;
     return -1;

@@ -499,6 +499,70 @@ def test_learned_index(sia, orc):
     ix.close()
 
 
+def test_learned_plan_forms_and_forced_fallback(sia, orc):
+    """The learned model as the plan's bound search (knob ub_learned): the batched search and
+    count through it equal the plain-halving and tree forms and the oracle.  With every leaf
+    window narrowed (test knob learned_shrink) the queries take the whole-leaf fallback and
+    the output is still identical.  A rebuilt index drops the model; destroying the index
+    before the model is safe."""
+    import torch
+    from superintervals_amd import autotune
+    print("[gpu] learned plan forms and forced fallback", flush=True)
+    s, e = sia.workload_intervals(4, 1_000_000, seed=43, sorted=False)
+    ix = _dev_index(sia, s, e)
+    model = sia.LearnedIndex(ix, 4096)
+    model.train(steps=200, batch=65536, lr=3e-3)
+    qs, qe = sia.workload_queries(4, 200_000, seed=44)
+    qs[:100] = np.iinfo(np.int32).min          # lower bound of the smallest value
+    oix = orc.build_index(s, e)
+    wd = orc.search_batch(oix, qs, qe, mode=0)
+    wv = orc.search_batch(oix, qs, qe, mode=1)
+    wc = orc.count_batch(oix, qs, qe)
+    saved = autotune.current()
+    try:
+        for knobs in ({"ub_tree": 0, "ub_learned": 0}, {"ub_tree": 1, "ub_learned": 0}, {"ub_learned": 1},
+                      {"ub_learned": 1, "range_skip": 0}, {"ub_learned": 1, "learned_shrink": 1 << 20},
+                      {"ub_learned": 1, "learned_shrink": 3}):
+            autotune.apply(dict({"ub_tree": 1, "range_skip": 1, "learned_shrink": 0}, **knobs))
+            print(f"[gpu] plan bounds {knobs}", flush=True)
+            assert np.array_equal(ix.count(qs, qe), wc), knobs
+            off, idx = ix.search(qs, qe, order="desc")
+            assert np.array_equal(off, wd[0]) and np.array_equal(idx.astype(np.int64), wd[1]), knobs
+            off, idx = ix.search(qs, qe, order="vector")
+            assert np.array_equal(off, wv[0]) and np.array_equal(idx.astype(np.int64), wv[1]), knobs
+        # the forced fallback, counted: identical to upper_bound with fallbacks > 0
+        starts = ix.download()["starts"]
+        want = np.searchsorted(starts, qe, side="right").astype(np.int64) - 1
+        xt = torch.from_numpy(qe).cuda()
+        for shrink in (0, 3, 1 << 20):
+            autotune.apply({"learned_shrink": shrink})
+            out = torch.empty(qe.size, dtype=torch.int32, device="cuda")
+            st = torch.zeros(3, dtype=torch.int64, device="cuda")
+            model.upper_bound_device(xt, out, st)
+            got = out.cpu().numpy().view(np.uint32).astype(np.int64)
+            got[got == 0xFFFFFFFF] = -1
+            assert np.array_equal(got, want), shrink
+            fb = int(st[0].item())
+            print(f"[gpu] learned_shrink {shrink}: fallbacks {fb}", flush=True)
+            if shrink == 0:
+                assert fb == 0
+            if shrink == 1 << 20:
+                assert fb >= int((want >= 0).sum())    # every query with a position took it
+    finally:
+        autotune.apply(saved)
+    # rebuilding the index drops the model: ub_learned then plans through the tree
+    ix.build(s, e)
+    with pytest.raises(sia.SIError):
+        model.upper_bound_device(xt, torch.empty(qe.size, dtype=torch.int32, device="cuda"))
+    try:
+        autotune.apply({"ub_learned": 1})
+        assert np.array_equal(ix.count(qs, qe), wc)
+    finally:
+        autotune.apply(saved)
+    ix.close()                   # the index first: the model forgets it
+    model.close()
+
+
 def test_walk_stats_match_oracle(sia, orc):
     import torch
     rec, s, e, qs, qe = load_golden("c1_sorted")
