@@ -213,6 +213,42 @@ int siSearchValuesFillPlannedBounded(const siIndex* ix, void* plan, size_t nq, c
 int siSearchNoAlloc(const siIndex* ix, const void* qs, const void* qe, size_t nq, int order,
                     uint32_t* counts, uint64_t* offsets, uint32_t* idx, uint64_t capacity, void* stream);
 
+/* ---- Set tables: batches over many resident indexes --------------------------
+ * No counterpart in the reference (its only multi-index structure is a map from sequence
+ * name to IntervalMap, examples/bed-intersect-si.rs:100-123).  A set table holds up to
+ * max_sets built int32 indexes in numbered slots; one batch may hold queries of any mix of
+ * slots (set_ids[q] = the slot of query q; an id >= max_sets is an empty set).  Results are
+ * each slot's own: every query's count and position list equal what siSearchOffsets /
+ * siSearchFill give on that slot's index alone.  Putting or evicting a slot touches no other
+ * slot.  The table holds device pointers of the indexes: put a slot again after rebuilding
+ * its index (calls on a table whose index was rebuilt since it was put return
+ * SI_ERR_NOT_BUILT), and do not destroy an index while batches that use it are in flight.
+ * Both calls are stream-ordered with no host read-back, so they can be captured in a graph;
+ * calls that share one plan must be ordered on one stream. */
+typedef struct siSetTable siSetTable;
+
+/* A table of max_sets slots (1 .. 4095) for indexes of coord_bytes (4: int32 only). */
+siSetTable* siSetTableCreate(int coord_bytes, uint32_t max_sets);
+void siSetTableDestroy(siSetTable* t);
+
+/* Puts a built index into slot (replacing what was there); ix = NULL empties the slot. */
+int siSetTablePut(siSetTable* t, uint32_t slot, const siIndex* ix);
+
+/* Device bytes of a plan for nq queries of this table. */
+size_t siSetTablePlanBytes(const siSetTable* t, size_t nq);
+
+/* Plans the batch (sorted by slot, then end) into plan and writes offsets[nq + 1] (and
+ * counts[nq] when not NULL) in input order, as siSearchOffsets per slot. */
+int siSetTableSearchOffsets(const siSetTable* t, const uint32_t* set_ids, const void* qs, const void* qe,
+                            size_t nq, uint32_t* counts, uint64_t* offsets, void* plan, size_t plan_bytes,
+                            void* stream);
+
+/* Fill pass of a planned batch: each query's positions in its slot's index, in the given
+ * order, at idx[offsets[q] ..), writing only slots below capacity (see
+ * siSearchFillPlannedBounded). */
+int siSetTableSearchFill(const siSetTable* t, void* plan, size_t nq, int order, const uint64_t* offsets,
+                         uint32_t* idx, uint64_t capacity, void* stream);
+
 /* ---- Batched queries on device buffers ------------------------------------
  * These plan the batch internally (stream-ordered scratch) and then run. */
 

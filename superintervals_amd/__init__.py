@@ -20,7 +20,7 @@ import numpy as np
 from . import _native as N
 from ._native import SI_ORDER_DESC, SI_ORDER_VECTOR, SI_INDEX_NONE, SIError
 
-__all__ = ["DeviceIndex", "LearnedIndex", "IntervalMap", "SIError", "SI_ORDER_DESC", "SI_ORDER_VECTOR",
+__all__ = ["DeviceIndex", "LearnedIndex", "SetTable", "IntervalMap", "SIError", "SI_ORDER_DESC", "SI_ORDER_VECTOR",
            "SI_INDEX_NONE", "workload_intervals", "workload_queries", "add_flags", "lib_path"]
 
 lib_path = N.LIB_PATH
@@ -364,6 +364,68 @@ class DeviceIndex:
     def values_fill_planned(self, plan_t, nq, offsets_t, vals_t, stream=None):
         N.check(N.lib.siSearchValuesFillPlanned(self._h, _ptr(plan_t), nq, _ptr(offsets_t), _ptr(vals_t),
                                                 _stream_of(stream)), "siSearchValuesFillPlanned")
+
+
+class SetTable:
+    """Many resident int32 indexes answered in one batch (si_device.h siSetTable): slot ->
+    DeviceIndex; every query names its slot.  Results equal each slot's own DeviceIndex."""
+
+    def __init__(self, max_sets: int):
+        h = N.lib.siSetTableCreate(4, max_sets)
+        if not h:
+            raise SIError(N.SI_ERR_ARG, "siSetTableCreate")
+        self._h = h
+        self.max_sets = max_sets
+        self.slots = [None] * max_sets       # keeps each DeviceIndex alive while it is in a slot
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.lib.siSetTableDestroy(self._h)
+            self._h = None
+        self.slots = []
+
+    def __del__(self):
+        self.close()
+
+    def put(self, slot: int, index):
+        """Put a built int32 DeviceIndex into slot (None empties it)."""
+        if index is not None and index.dtype != np.dtype(np.int32):
+            raise TypeError("set tables hold int32 indexes")
+        N.check(N.lib.siSetTablePut(self._h, slot, index.handle if index is not None else None), "siSetTablePut")
+        self.slots[slot] = index
+
+    def plan_bytes(self, nq: int) -> int:
+        return int(N.lib.siSetTablePlanBytes(self._h, nq))
+
+    def offsets_device(self, set_ids_t, qs_t, qe_t, plan_t, offsets_t, counts_t=None, stream=None):
+        N.check(N.lib.siSetTableSearchOffsets(self._h, _ptr(set_ids_t), _ptr(qs_t), _ptr(qe_t), qs_t.numel(),
+                                              _ptr(counts_t), _ptr(offsets_t), _ptr(plan_t),
+                                              plan_t.numel() * plan_t.element_size(), _stream_of(stream)),
+                "siSetTableSearchOffsets")
+
+    def fill_device(self, plan_t, nq, offsets_t, idx_t, order="desc", capacity=None, stream=None):
+        cap = idx_t.numel() if capacity is None else int(capacity)
+        N.check(N.lib.siSetTableSearchFill(self._h, _ptr(plan_t), nq, _ORDERS[order], _ptr(offsets_t), _ptr(idx_t),
+                                           cap, _stream_of(stream)), "siSetTableSearchFill")
+
+    def search(self, set_ids, qs, qe, order="desc"):
+        """Host-buffer convenience: CSR (offsets uint64, positions uint32) of the batch."""
+        import torch
+        dev = torch.device("cuda:0")
+        nq = int(np.asarray(qs).size)
+        sid = torch.from_numpy(np.ascontiguousarray(set_ids, np.uint32).view(np.int32)).to(dev)
+        tqs = torch.from_numpy(np.ascontiguousarray(qs, np.int32)).to(dev)
+        tqe = torch.from_numpy(np.ascontiguousarray(qe, np.int32)).to(dev)
+        plan = torch.empty(max(self.plan_bytes(nq), 256), dtype=torch.uint8, device=dev)
+        off = torch.empty(nq + 1, dtype=torch.int64, device=dev)
+        self.offsets_device(sid, tqs, tqe, plan, off)
+        torch.cuda.synchronize()
+        total = int(off[-1].item())
+        idx = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+        if total:
+            self.fill_device(plan, nq, off, idx, order=order, capacity=total)
+        torch.cuda.synchronize()
+        return off.cpu().numpy().astype(np.uint64), idx[:total].cpu().numpy().view(np.uint32)
 
 
 class LearnedIndex:

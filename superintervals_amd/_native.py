@@ -87,6 +87,12 @@ _SIGS = {
     "siSearchFillPlanned": (_i, [_vp, _vp, _sz, _i, _vp, _vp, _vp]),
     "siPlanCheck": (_i, [_vp, _vp, _vp]),
     "siSearchNoAlloc": (_i, [_vp, _vp, _vp, _sz, _i, _vp, _vp, _vp, C.c_uint64, _vp]),
+    "siSetTableCreate": (_vp, [_i, C.c_uint32]),
+    "siSetTableDestroy": (None, [_vp]),
+    "siSetTablePut": (_i, [_vp, C.c_uint32, _vp]),
+    "siSetTablePlanBytes": (_sz, [_vp, _sz]),
+    "siSetTableSearchOffsets": (_i, [_vp, _vp, _vp, _vp, _sz, _vp, _vp, _vp, _sz, _vp]),
+    "siSetTableSearchFill": (_i, [_vp, _vp, _sz, _i, _vp, _vp, C.c_uint64, _vp]),
     "siSearchValuesFillPlanned": (_i, [_vp, _vp, _sz, _vp, _vp, _vp]),
     "siSearchFillPlannedBounded": (_i, [_vp, _vp, _sz, _i, _vp, _vp, C.c_uint64, _vp]),
     "siSearchValuesFillPlannedBounded": (_i, [_vp, _vp, _sz, _vp, _vp, C.c_uint64, _vp]),

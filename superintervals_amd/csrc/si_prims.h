@@ -372,6 +372,11 @@ __global__ void k_zero_u32(uint32_t* __restrict__ p, size_t n) {
     if (i < n) p[i] = 0;
 }
 
+__global__ void k_zero_u64(uint64_t* __restrict__ p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 // Stable sort of (keys, vals) by key digits [digit_lo, digit_hi) (8-bit digits,
 // least significant first).  Digits that are equal for every key are skipped, which
 // needs one host read-back of the digit histograms; skip_constant = false sorts every

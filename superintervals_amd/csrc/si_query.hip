@@ -63,6 +63,7 @@ struct PlanView {
     S* e;                // query end of sorted position k
     uint32_t* idx;       // upper_bound of sorted position k
     uint32_t* lb;        // first position whose start is >= the query start (lower_bound)
+    uint32_t* set;       // plans over a set table: the set of sorted position k
     HeavyEntry* heavy;   // parked walks (see HeavyEntry)
 };
 
@@ -106,6 +107,38 @@ struct PlanLayout {
     }
 };
 
+// Levels of the upper_bound search tree (level 0 = starts), passed by value.
+template <typename S>
+struct UbTree {
+    const S* lv[MAX_LEVELS];     // lv[0] = starts
+    uint32_t size[MAX_LEVELS];
+    int nlv;                     // levels above the starts
+};
+
+// One resident index of a set table (siSetTable): what the batch kernels read of it.
+template <typename S>
+struct SetView {
+    IndexView<S> v;               // sorted arrays (n = 0: an empty slot)
+    const int32_t* values;        // payload in sorted order, or null
+    UbTree<S> ub;                 // search tree over the starts
+    const S* ub_top;
+    uint32_t ub_h;
+    UbTree<S> eb;                 // search tree over the sorted ends
+    const S* eb_top;
+    uint32_t eb_h;
+    TreeDesc<S> td;               // max tree over the ends
+    uint32_t rank_exact;          // 1 when no interval has start > end
+};
+
+// Tiles of a plan over several sets: the sorted queries of set g are [sbeg[g], sbeg[g] +
+// scnt[g]) and its tiles are tbeg[g] .. tbeg[g + 1] - 1; tbeg[nsets] is the number of tiles.
+struct MultiTiles {
+    const uint32_t* scnt;
+    const uint32_t* sbeg;
+    const uint32_t* tbeg;
+    uint32_t nsets;
+};
+
 template <typename S>
 PlanView<S> plan_view(void* base, size_t nq) {
     PlanLayout L(sizeof(S), nq);
@@ -117,6 +150,7 @@ PlanView<S> plan_view(void* base, size_t nq) {
     v.e = (S*)(p + L.off_e);
     v.idx = (uint32_t*)(p + L.off_idx);
     v.lb = (uint32_t*)(p + L.off_lb);
+    v.set = nullptr;
     v.heavy = (HeavyEntry*)(p + L.off_heavy);
     return v;
 }
@@ -506,8 +540,10 @@ static_assert(SI_WALK_WIN32 % 16 == 0 && SI_WALK_WIN64 % 16 == 0, "window sizes 
 
 constexpr int T_THREADS = 512;   // 8 waves per workgroup, two workgroups per CU
 
-template <typename S, int MODE>
-__global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<S> ix, PlanView<S> pv, uint32_t nq,
+// MULTI: the plan covers several resident indexes (a set table): each tile holds queries of
+// one set, whose arrays the tile reads (sets[g]), and positions are the set's own.
+template <typename S, int MODE, bool MULTI = false>
+__global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<S> ix_arg, PlanView<S> pv, uint32_t nq,
                                                     uint32_t tile, uint32_t ntiles, uint32_t lane_steps,
                                                     bool use_lb,
                                                     uint32_t* __restrict__ tile_counter,
@@ -515,13 +551,16 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
                                                     uint64_t* __restrict__ counts64,
                                                     const uint64_t* __restrict__ offsets,
                                                     uint32_t* __restrict__ out_idx,
-                                                    const int32_t* __restrict__ vals,
-                                                    int32_t* __restrict__ out_val, uint64_t cap) {
+                                                    const int32_t* __restrict__ vals_arg,
+                                                    int32_t* __restrict__ out_val, uint64_t cap,
+                                                    const SetView<S>* __restrict__ sets, MultiTiles mt) {
     constexpr int W = Line<S>::W;
     constexpr uint32_t WIN = TileCfg<S>::WIN;
     __shared__ __align__(16) S s_end[WIN];
     __shared__ __align__(16) uint32_t s_ex[WIN];   // exit targets (IndexView::exitp)
-    __shared__ uint32_t s_tile, s_next, s_whi1;
+    __shared__ uint32_t s_tile, s_next, s_whi1, s_set, s_t0, s_t1;
+    IndexView<S> ix = ix_arg;
+    const int32_t* __restrict__ vals = vals_arg;
     // per-wave emission slots: 64 + one for the segment carried between chunks
     __shared__ EmitSlot s_emit[(MODE == W_COUNT) ? 1 : T_THREADS / WAVE * (WAVE + 1)];
     __shared__ uint32_t s_kbit[(MODE == W_COUNT || !SI_EMIT_LUT) ? 1 : 256];   // byte set-bit table
@@ -543,13 +582,42 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
     while (true) {
         // ---- take the next tile (workgroup-level work stealing)
         if (threadIdx.x == 0) {
-            s_tile = atomicAdd(tile_counter, 1u);
+            const uint32_t tt = atomicAdd(tile_counter, 1u);
+            s_tile = tt;
             s_whi1 = 0u;
+            if (MULTI) {
+                // the set of tile tt: the last g with tbeg[g] <= tt (sets without queries
+                // have no tiles); tiles past the last are NONE
+                const uint32_t total = mt.tbeg[mt.nsets];
+                uint32_t lo = 0, hi = mt.nsets;   // tbeg[lo] <= tt < tbeg[hi]
+                if (tt < total) {
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (mt.tbeg[mid] <= tt) lo = mid;
+                        else hi = mid;
+                    }
+                    s_set = lo;
+                    s_t0 = mt.sbeg[lo] + (tt - mt.tbeg[lo]) * tile;
+                    s_t1 = min(s_t0 + tile, mt.sbeg[lo] + mt.scnt[lo]);
+                } else {
+                    s_tile = NONE;
+                }
+            }
         }
         __syncthreads();
         const uint32_t t = s_tile;
         if (t >= ntiles) break;
-        const uint32_t t0 = t * tile, t1 = min(t0 + tile, nq);
+        uint32_t t0, t1;
+        if (MULTI) {
+            const uint32_t g = __builtin_amdgcn_readfirstlane(s_set);
+            t0 = __builtin_amdgcn_readfirstlane(s_t0);
+            t1 = __builtin_amdgcn_readfirstlane(s_t1);
+            ix = sets[g].v;
+            vals = sets[g].values;
+        } else {
+            t0 = t * tile;
+            t1 = min(t0 + tile, nq);
+        }
         // the window ends at the tile's largest upper_bound (the plan's order by end need
         // not be exact): max of idx + 1 over the tile, where NONE + 1 = 0 drops queries
         // that have no start <= their end
@@ -932,14 +1000,18 @@ __device__ __forceinline__ void wave_walk(const IndexView<S>& ix, const TreeDesc
 
 // Continuation of walks the lane walk parked (over its step budget): one wave per parked
 // query, taken from a global counter.
-template <typename S, int MODE>
-__global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, TreeDesc<S> td, PlanView<S> pv,
+template <typename S, int MODE, bool MULTI = false>
+__global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix_arg, TreeDesc<S> td_arg, PlanView<S> pv,
                                                           uint32_t* __restrict__ counts,
                                                           uint64_t* __restrict__ counts64,
                                                           const uint64_t* __restrict__ offsets,
                                                           uint32_t* __restrict__ out_idx,
-                                                          const int32_t* __restrict__ vals,
-                                                          int32_t* __restrict__ out_val, uint64_t cap) {
+                                                          const int32_t* __restrict__ vals_arg,
+                                                          int32_t* __restrict__ out_val, uint64_t cap,
+                                                          const SetView<S>* __restrict__ sets) {
+    IndexView<S> ix = ix_arg;
+    TreeDesc<S> td = td_arg;
+    const int32_t* __restrict__ vals = vals_arg;
     const int lane = lane_id();
     const uint32_t nheavy = __builtin_amdgcn_readfirstlane(pv.counter[1]);
     if (nheavy == 0) return;   // nothing parked: no wave touches the shared counter
@@ -950,6 +1022,12 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix, TreeD
         if (h >= nheavy) break;
         const HeavyEntry he = pv.heavy[h];
         const uint32_t k = __builtin_amdgcn_readfirstlane(he.k);
+        if (MULTI) {   // the parked query's set
+            const uint32_t g = __builtin_amdgcn_readfirstlane(pv.set[k]);
+            ix = sets[g].v;
+            td = sets[g].td;
+            vals = sets[g].values;
+        }
         const uint32_t q = __builtin_amdgcn_readfirstlane(pv.q[k]);
         const S s = pv.s[k];
         const uint32_t cur = __builtin_amdgcn_readfirstlane(he.cur);
@@ -991,14 +1069,6 @@ __device__ __forceinline__ int64_t last_le_in(const S* __restrict__ v, int64_t l
     }
     return (int64_t)(base - v) - ((*base <= x) ? 0 : 1);
 }
-
-// Levels of the upper_bound search tree (level 0 = starts), passed by value.
-template <typename S>
-struct UbTree {
-    const S* lv[MAX_LEVELS];     // lv[0] = starts
-    uint32_t size[MAX_LEVELS];
-    int nlv;                     // levels above the starts
-};
 
 // Last position with key <= x, or NONE, through the tree (s_top = its top level in LDS).
 template <typename S>
@@ -1287,7 +1357,7 @@ int launch_walk(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint
     LAUNCH_CHECK();
     k_walk<S, MODE><<<grid, T_THREADS, 0, st>>>(
         ix->view<S>(), pv, (uint32_t)nq, tile, ntiles, g_tune.lane_steps, range_skip_on(ix), pv.counter, counts,
-        counts64, offsets, out_idx, ix->d_values, out_val, cap);
+        counts64, offsets, out_idx, ix->d_values, out_val, cap, nullptr, MultiTiles{nullptr, nullptr, nullptr, 0u});
     LAUNCH_CHECK();
     // parked (heavy) walks: one wave each, resident grid, exits at once when none were parked
     static int cus = 0;
@@ -1300,7 +1370,8 @@ int launch_walk(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint
     IndexView<S> hv = ix->view<S>();
     if (!g_tune.heavy_ends32) hv.ends32 = nullptr;
     k_walk_heavy<S, MODE><<<(unsigned)cus * 8u, Q_THREADS, 0, st>>>(hv, ix->tree<S>(), pv, counts, counts64,
-                                                                   offsets, out_idx, ix->d_values, out_val, cap);
+                                                                   offsets, out_idx, ix->d_values, out_val, cap,
+                                                                   nullptr);
     LAUNCH_CHECK();
     return SI_OK;
 }
@@ -1791,6 +1862,313 @@ SI_EXPORT int siWalkStats(const siIndex* ix, const void* qs, const void* qe, siz
 }
 
 // ============================================================================
+// Set tables: batches over many resident indexes
+// ============================================================================
+// A set table holds the device views of up to max_sets built indexes (slots), plus one
+// always-empty slot for out-of-range set numbers.  A batch gives every query its slot; the
+// plan sorts the batch by (slot, end), finds each query's bounds and its rank-form count
+// through its own set's search trees, and lays out tiles that never mix sets, so the walk
+// kernels run unchanged on each tile's set and write the set's own positions.  Putting or
+// evicting a set rewrites one slot; the other sets and their indexes are untouched.
+struct siSetTable {
+    int cb = 4;
+    uint32_t nslots = 0;                       // max_sets + 1 (the last is always empty)
+    void* d_views = nullptr;                   // nslots SetView<int32_t>
+    std::vector<const siIndex*> sets;          // the index in each slot (not owned)
+    std::vector<uint64_t> gens;                // its build generation when it was put
+};
+
+namespace {
+
+SetView<int32_t> set_view_of(const siIndex* ix) {
+    SetView<int32_t> v;
+    std::memset(&v, 0, sizeof(v));
+    if (!ix || ix->n == 0) return v;
+    v.v = ix->view<int32_t>();
+    v.values = ix->d_values;
+    v.ub = ub_tree_of<int32_t>(ix->ub, (const int32_t*)ix->d_starts, ix->n);
+    v.ub_top = (const int32_t*)ix->ub.d_top;
+    v.ub_h = ix->ub.top_h;
+    v.eb = ub_tree_of<int32_t>(ix->eb, (const int32_t*)ix->d_ends_sorted, ix->n);
+    v.eb_top = (const int32_t*)ix->eb.d_top;
+    v.eb_h = ix->eb.top_h;
+    v.td = ix->tree<int32_t>();
+    v.rank_exact = ix->n_inverted == 0 ? 1u : 0u;
+    return v;
+}
+
+// Byte layout of a set-table plan: a plan of nq queries, then per sorted query its slot, and
+// per slot its query count, its first sorted query and its first tile (nslots + 1 each).
+struct MultiLayout {
+    PlanLayout base;
+    size_t off_set, off_scnt, off_sbeg, off_tbeg, total;
+    MultiLayout(size_t nq, uint32_t nslots) : base(4, nq) {
+        off_set = base.total;
+        off_scnt = off_set + align256(4 * nq);
+        off_sbeg = off_scnt + align256(4 * ((size_t)nslots + 1));
+        off_tbeg = off_sbeg + align256(4 * ((size_t)nslots + 1));
+        total = off_tbeg + align256(4 * ((size_t)nslots + 1));
+    }
+};
+
+constexpr int MK_THREADS = 256;
+constexpr int MK_ITEMS = 8;   // queries per thread in the key kernel
+
+// Sort keys (slot << 32 | biased end), payload = input number; per-slot query counts through
+// an LDS histogram.  Slots past the table map to the empty slot.
+__global__ __launch_bounds__(MK_THREADS) void k_multi_keys(const uint32_t* __restrict__ set_ids,
+                                                           const int32_t* __restrict__ qe, size_t nq,
+                                                           uint32_t nslots, uint64_t* __restrict__ key,
+                                                           uint32_t* __restrict__ val, uint32_t* __restrict__ scnt) {
+    extern __shared__ uint32_t s_hist[];
+    for (uint32_t b = threadIdx.x; b < nslots; b += MK_THREADS) s_hist[b] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * MK_THREADS * MK_ITEMS;
+    for (int j = 0; j < MK_ITEMS; ++j) {
+        const size_t k = base + (size_t)j * MK_THREADS + threadIdx.x;
+        if (k < nq) {
+            uint32_t g = set_ids[k];
+            if (g >= nslots - 1) g = nslots - 1;
+            key[k] = ((uint64_t)g << 32) | (uint32_t)((uint32_t)qe[k] ^ 0x80000000u);
+            val[k] = (uint32_t)k;
+            atomicAdd(&s_hist[g], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nslots; b += MK_THREADS)
+        if (s_hist[b]) atomicAdd(&scnt[b], s_hist[b]);
+}
+
+// Per sorted query: its slot, start, end; upper_bound and lower_bound through its set's
+// starts tree; its count in input order, the rank form through the set's sorted-ends tree
+// (the walk when the set or the query has start > end).
+__global__ __launch_bounds__(Q_THREADS) void k_multi_finish(const SetView<int32_t>* __restrict__ sets,
+                                                            const uint64_t* __restrict__ key,
+                                                            const uint32_t* __restrict__ val,
+                                                            const int32_t* __restrict__ qs, size_t nq,
+                                                            PlanView<int32_t> pv, uint32_t* __restrict__ counts) {
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) {
+        pv.counter[0] = 0;
+        pv.counter[4] = 1u;   // lower bounds present (NONE where the set has start > end)
+    }
+    if (k >= nq) return;
+    const uint64_t kk = key[k];
+    const uint32_t q = val[k];
+    const uint32_t g = (uint32_t)(kk >> 32);
+    const int32_t e = (int32_t)((uint32_t)kk ^ 0x80000000u);
+    const int32_t s = qs[q];
+    pv.q[k] = q;
+    pv.s[k] = s;
+    pv.e[k] = e;
+    pv.set[k] = g;
+    const SetView<int32_t>& sv = sets[g];
+    uint32_t ub = NONE, lb = NONE, c = 0;
+    if (sv.v.n) {
+        ub = ub_tree_pos(sv.ub, sv.ub_top, sv.ub_h, e);
+        if (sv.rank_exact) lb = (s == INT32_MIN) ? 0u : ub_tree_pos(sv.ub, sv.ub_top, sv.ub_h, s - 1) + 1u;
+        if (ub != NONE) {
+            if (sv.rank_exact && s <= e) {
+                const uint32_t below = (s == INT32_MIN) ? 0u : ub_tree_pos(sv.eb, sv.eb_top, sv.eb_h, s - 1) + 1u;
+                c = ub + 1u - below;
+            } else {
+                c = walk_count(sv.v, ub, s);
+            }
+        }
+    }
+    pv.idx[k] = ub;
+    pv.lb[k] = lb;
+    counts[q] = c;
+}
+
+// First sorted query and first tile of every slot (one thread: at most a few thousand slots).
+__global__ void k_multi_tiles(const uint32_t* __restrict__ scnt, uint32_t nslots, uint32_t tile,
+                              uint32_t* __restrict__ sbeg, uint32_t* __restrict__ tbeg) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    uint32_t a = 0, b = 0;
+    for (uint32_t g = 0; g < nslots; ++g) {
+        sbeg[g] = a;
+        tbeg[g] = b;
+        a += scnt[g];
+        b += (scnt[g] + tile - 1u) / tile;
+    }
+    sbeg[nslots] = a;
+    tbeg[nslots] = b;
+}
+
+PlanView<int32_t> multi_view(void* plan, size_t nq, uint32_t nslots) {
+    const MultiLayout L(nq, nslots);
+    PlanView<int32_t> pv = plan_view<int32_t>(plan, nq);
+    pv.set = (uint32_t*)((char*)plan + L.off_set);
+    return pv;
+}
+
+MultiTiles multi_tiles(void* plan, size_t nq, uint32_t nslots) {
+    const MultiLayout L(nq, nslots);
+    char* p = (char*)plan;
+    return MultiTiles{(const uint32_t*)(p + L.off_scnt), (const uint32_t*)(p + L.off_sbeg),
+                      (const uint32_t*)(p + L.off_tbeg), nslots};
+}
+
+int check_table(const siSetTable* t) {
+    if (!t) return si_set_error(SI_ERR_ARG, "null set table");
+    for (uint32_t g = 0; g + 1 < t->nslots; ++g) {
+        const siIndex* ix = t->sets[g];
+        if (ix && (!ix->built || ix->generation != t->gens[g]))
+            return si_set_error(SI_ERR_NOT_BUILT, "the index in slot %u was rebuilt or cleared since it was put", g);
+    }
+    return SI_OK;
+}
+
+// Tile size of the multi walk: the tuned tile (tiles never mix sets).
+uint32_t multi_tile() { return std::max<uint32_t>(POOL, (g_tune.tile + POOL - 1) / POOL * POOL); }
+
+template <int MODE>
+int launch_walk_multi(const siSetTable* tb, void* plan, size_t nq, const uint64_t* offsets, uint32_t* out_idx,
+                      hipStream_t st, uint64_t cap) {
+    if (nq == 0) return SI_OK;
+    PlanView<int32_t> pv = multi_view(plan, nq, tb->nslots);
+    const MultiTiles mt = multi_tiles(plan, nq, tb->nslots);
+    static int cus = 0, per = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_walk<int32_t, MODE, true>, T_THREADS, 0);
+        cus = std::max(cus, 1);
+        per = std::max(per, 1);
+    }
+    const int wgs = (g_tune.fill_wgs > 0) ? std::min(g_tune.fill_wgs, per) : per;
+    const uint32_t tile = multi_tile();
+    const size_t tiles_max = (nq + tile - 1) / tile + tb->nslots;
+    const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>((size_t)cus * wgs, tiles_max));
+    k_zero_u32<<<1, 64, 0, st>>>(pv.counter, 4);
+    LAUNCH_CHECK();
+    const SetView<int32_t>* sets = (const SetView<int32_t>*)tb->d_views;
+    k_walk<int32_t, MODE, true><<<grid, T_THREADS, 0, st>>>(
+        IndexView<int32_t>{}, pv, (uint32_t)nq, tile, NONE - 1u, g_tune.lane_steps, true, pv.counter, nullptr,
+        nullptr, offsets, out_idx, nullptr, nullptr, cap, sets, mt);
+    LAUNCH_CHECK();
+    k_walk_heavy<int32_t, MODE, true><<<(unsigned)cus * 8u, Q_THREADS, 0, st>>>(
+        IndexView<int32_t>{}, TreeDesc<int32_t>{}, pv, nullptr, nullptr, offsets, out_idx, nullptr, nullptr, cap,
+        sets);
+    LAUNCH_CHECK();
+    return SI_OK;
+}
+
+}  // namespace
+
+SI_EXPORT siSetTable* siSetTableCreate(int coord_bytes, uint32_t max_sets) {
+    if (coord_bytes != 4) {
+        si_set_error(SI_ERR_UNSUPPORTED, "set tables hold int32 indexes");
+        return nullptr;
+    }
+    if (max_sets < 1 || max_sets > 4095) {   // the key kernel's per-slot histogram lives in LDS
+        si_set_error(SI_ERR_ARG, "max_sets must be in [1, 4095]");
+        return nullptr;
+    }
+    siSetTable* t = new siSetTable();
+    t->cb = 4;
+    t->nslots = max_sets + 1;
+    t->sets.assign(t->nslots, nullptr);
+    t->gens.assign(t->nslots, 0);
+    const size_t bytes = (size_t)t->nslots * sizeof(SetView<int32_t>);
+    std::vector<char> zero(bytes, 0);
+    if (hipMalloc(&t->d_views, bytes) != hipSuccess ||
+        hipMemcpy(t->d_views, zero.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        si_set_error(SI_ERR_NOMEM, "set table allocation");
+        (void)hipFree(t->d_views);
+        delete t;
+        return nullptr;
+    }
+    return t;
+}
+
+SI_EXPORT void siSetTableDestroy(siSetTable* t) {
+    if (!t) return;
+    (void)hipFree(t->d_views);
+    delete t;
+}
+
+SI_EXPORT int siSetTablePut(siSetTable* t, uint32_t slot, const siIndex* ix) {
+    if (!t) return si_set_error(SI_ERR_ARG, "null set table");
+    if (slot + 1 >= t->nslots) return si_set_error(SI_ERR_ARG, "slot %u out of range (max_sets %u)", slot, t->nslots - 1);
+    if (ix) {
+        int rc = check_ix(ix);
+        if (rc) return rc;
+        if (ix->cb != t->cb) return si_set_error(SI_ERR_ARG, "the index's coordinate width differs from the table's");
+    }
+    const SetView<int32_t> v = set_view_of(ix);
+    HIP_TRY(hipMemcpy((char*)t->d_views + (size_t)slot * sizeof(v), &v, sizeof(v), hipMemcpyHostToDevice));
+    t->sets[slot] = ix;
+    t->gens[slot] = ix ? ix->generation : 0;
+    return SI_OK;
+}
+
+SI_EXPORT size_t siSetTablePlanBytes(const siSetTable* t, size_t nq) {
+    return t ? MultiLayout(nq, t->nslots).total : 0;
+}
+
+SI_EXPORT int siSetTableSearchOffsets(const siSetTable* t, const uint32_t* set_ids, const void* qs, const void* qe,
+                                      size_t nq, uint32_t* counts, uint64_t* offsets, void* plan, size_t plan_bytes,
+                                      void* stream) {
+    int rc = check_table(t);
+    if (rc) return rc;
+    if (!offsets || !plan) return si_set_error(SI_ERR_ARG, "null offsets/plan");
+    if (nq >= (size_t)0x7FFFFFFF) return si_set_error(SI_ERR_ARG, "batch of %zu queries exceeds 2^31-1", nq);
+    const MultiLayout L(nq, t->nslots);
+    if (plan_bytes < L.total) return si_set_error(SI_ERR_ARG, "plan buffer too small: %zu < %zu bytes", plan_bytes, L.total);
+    hipStream_t st = (hipStream_t)stream;
+    if (nq == 0) {
+        k_zero_u64<<<1, 64, 0, st>>>(offsets, 1);
+        LAUNCH_CHECK();
+        return SI_OK;
+    }
+    if (!set_ids || !qs || !qe) return si_set_error(SI_ERR_ARG, "null queries");
+    char* p = (char*)plan;
+    uint64_t* k0 = (uint64_t*)(p + L.base.off_k0);
+    uint64_t* k1 = (uint64_t*)(p + L.base.off_k1);
+    uint32_t* v0 = (uint32_t*)(p + L.base.off_v0);
+    uint32_t* v1 = (uint32_t*)(p + L.base.off_v1);
+    RsTemp rt = rs_temp_carve(p + L.base.off_rs, nq);
+    uint32_t* scnt = (uint32_t*)(p + L.off_scnt);
+    k_zero_u32<<<grid_for(t->nslots, 256), 256, 0, st>>>(scnt, t->nslots);
+    LAUNCH_CHECK();
+    const unsigned gk = grid_for(nq, MK_THREADS * MK_ITEMS);
+    k_multi_keys<<<gk, MK_THREADS, t->nslots * sizeof(uint32_t), st>>>(set_ids, (const int32_t*)qe, nq, t->nslots, k0,
+                                                                       v0, scnt);
+    LAUNCH_CHECK();
+    // sort by (slot, end): the end's digits (the lowest plan_skip_low left unsorted) and the
+    // slot's one or two; no host read-back, so the batch can be captured in a graph
+    const int slot_digits = (t->nslots <= 256) ? 1 : 2;
+    uint64_t* ks;
+    uint32_t* vs;
+    rc = radix_sort_pairs(k0, v0, k1, v1, nq, g_tune.plan_skip_low, 4 + slot_digits, rt, &ks, &vs, st, false);
+    if (rc) return rc;
+    PlanView<int32_t> pv = multi_view(plan, nq, t->nslots);
+    uint32_t* c32 = counts ? counts : (uint32_t*)(p + L.base.off_heavy);
+    k_multi_finish<<<grid_for(nq, Q_THREADS), Q_THREADS, 0, st>>>((const SetView<int32_t>*)t->d_views, ks, vs,
+                                                                 (const int32_t*)qs, nq, pv, c32);
+    LAUNCH_CHECK();
+    k_multi_tiles<<<1, 64, 0, st>>>(scnt, t->nslots, multi_tile(), (uint32_t*)(p + L.off_sbeg),
+                                    (uint32_t*)(p + L.off_tbeg));
+    LAUNCH_CHECK();
+    uint64_t* ws = (uint64_t*)(p + L.base.off_scratch);
+    return device_excl_scan<uint32_t, uint64_t>(c32, nq, offsets, offsets + nq, ws, st);
+}
+
+SI_EXPORT int siSetTableSearchFill(const siSetTable* t, void* plan, size_t nq, int order, const uint64_t* offsets,
+                                   uint32_t* idx, uint64_t capacity, void* stream) {
+    int rc = check_table(t);
+    if (rc) return rc;
+    if (order != SI_ORDER_DESC && order != SI_ORDER_VECTOR) return si_set_error(SI_ERR_ARG, "bad order %d", order);
+    if (nq && (!plan || !offsets || (!idx && capacity))) return si_set_error(SI_ERR_ARG, "null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    if (order == SI_ORDER_DESC) return launch_walk_multi<W_DESC>(t, plan, nq, offsets, idx, st, capacity);
+    return launch_walk_multi<W_VECTOR>(t, plan, nq, offsets, idx, st, capacity);
+}
+
+// ============================================================================
 // Search without allocation
 // ============================================================================
 // siSearchNoAlloc keeps its plan in a file-scope device array instead of memory it
@@ -1824,10 +2202,6 @@ int scratch_ptr(void** out) {
     return SI_OK;
 }
 
-__global__ void k_zero_u64(uint64_t* __restrict__ p, size_t n) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
 }  // namespace
 
 SI_EXPORT int siSearchNoAlloc(const siIndex* ix, const void* qs, const void* qe, size_t nq, int order,
