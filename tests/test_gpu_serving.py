@@ -2,7 +2,8 @@
 
 Each request's result in a Poisson mix of requests over several resident interval sets
 must equal the result of the same request alone (tolerance 0: integer outputs), here the
-CPU oracle's answer on that set.
+CPU oracle's answer on that set.  Sets are int32 indexes in the slots of a set table;
+batches replay captured graphs per shape.
 """
 import numpy as np
 import pytest
@@ -21,9 +22,9 @@ def setup():
     rng = np.random.default_rng(5)
     sizes = serving.log_uniform_int(rng, 20_000, 200_000, 6)
     sets = [sia.workload_intervals(2, int(n), seed=100 + k, sorted=bool(k % 2)) for k, n in enumerate(sizes)]
-    res = serving.ResidentSet(sets)
+    res = serving.ResidentSets(sets, max_sets=16)
     O = oracle.Oracle()
-    oix = [O.build_index(s, e) for s, e in sets]
+    oix = {k: O.build_index(s, e) for k, (s, e) in enumerate(sets)}
     reqs = []
     for k in range(60):
         i = int(rng.integers(0, len(sets)))
@@ -33,10 +34,11 @@ def setup():
     return sia, serving, res, O, oix, reqs
 
 
-def _check(O, oix, reqs, done):
-    assert len(done) == len(reqs)
+def _check(O, oix, reqs, done, rids=None):
     by_id = {r.rid: r for r in done}
-    for rid, (i, qs, qe, kind) in enumerate(reqs):
+    rids = list(range(len(reqs))) if rids is None else rids
+    assert len(done) == len(rids)
+    for rid, (i, qs, qe, kind) in zip(rids, reqs):
         r = by_id[rid]
         assert r.index_id == i
         if kind == "count":
@@ -47,17 +49,6 @@ def _check(O, oix, reqs, done):
             assert np.array_equal(r.positions().astype(np.int64), idx), rid
 
 
-def test_resident_set_positions(setup):
-    sia, serving, res, O, oix, reqs = setup
-    a = res.index.download()
-    for k in range(len(oix)):
-        b0, b1 = res.base[k], res.base[k + 1]
-        assert np.array_equal(a["starts"][b0:b1] - k * serving.SHIFT, oix[k]["starts"])
-        br = a["branch"][b0:b1]
-        want = oix[k]["branch"]
-        assert np.array_equal(np.where(br < 0, -1, br - b0), want)
-
-
 def test_mix_equals_alone(setup):
     sia, serving, res, O, oix, reqs = setup
     print("[gpu] serving: Poisson mix, large budget", flush=True)
@@ -66,6 +57,7 @@ def test_mix_equals_alone(setup):
     lat, wall = serving.serve_stream(srv, reqs, arrivals, timeout_s=120)
     _check(O, oix, reqs, srv.completed)
     assert srv.stats["batches1"] >= 1 and srv.in_use == 0
+    assert srv.stats["graphs"] >= 2
     assert np.all(lat >= 0)
 
 
@@ -81,4 +73,50 @@ def test_tight_budget_queues_but_stays_exact(setup):
     assert srv.stats["batches1"] > 1          # the budget forced several batches
     assert srv.in_use == 0
     with pytest.raises(MemoryError):
-        srv.submit(0, np.zeros(1 << 20, np.int32), np.zeros(1 << 20, np.int32))
+        srv.submit(0, np.zeros(1 << 24, np.int32), np.zeros(1 << 24, np.int32))
+
+
+def test_larger_mix_with_sets_added_and_evicted(setup):
+    """More and larger sets (up to 1.5M intervals, one unsorted with duplicates), larger
+    requests, and sets added and evicted while the server runs; every result exact."""
+    sia, serving, res0, O, oix0, _ = setup
+    print("[gpu] serving: larger mix, add / evict", flush=True)
+    rng = np.random.default_rng(77)
+    sizes = serving.log_uniform_int(rng, 50_000, 1_500_000, 10)
+    sets = {k: sia.workload_intervals(2, int(n), seed=300 + k, sorted=bool(k % 3)) for k, n in enumerate(sizes)}
+    res = serving.ResidentSets(max_sets=32)
+    for k, (s, e) in sets.items():
+        res.add(k, s, e)
+    oix = {k: O.build_index(s, e) for k, (s, e) in sets.items()}
+    srv = serving.Server(res, hbm_budget_bytes=2 << 30, streams=4)
+
+    def make(k, slots):
+        i = int(rng.choice(slots))
+        m = int(serving.log_uniform_int(rng, 1, 200_000))
+        qs, qe = sia.workload_queries(2, m, seed=900 + k)
+        return (i, qs, qe, "count" if k % 11 == 5 else "search")
+
+    first = [make(k, list(sets)) for k in range(40)]
+    arrivals = serving.poisson_arrivals(400.0, len(first), seed=3)
+    serving.serve_stream(srv, first, arrivals, timeout_s=300)
+    _check(O, oix, first, srv.completed)
+    # evict two sets, add a new one in a free slot and replace another
+    srv.evict_set(3)
+    srv.evict_set(7)
+    with pytest.raises(ValueError):
+        srv.submit(3, np.zeros(4, np.int32), np.ones(4, np.int32))
+    s_new, e_new = sia.workload_intervals(2, 800_000, seed=999, sorted=False)
+    srv.add_set(20, s_new, e_new)
+    s_rep, e_rep = sia.workload_intervals(2, 300_000, seed=998, sorted=True)
+    srv.add_set(1, s_rep, e_rep)
+    oix[20] = O.build_index(s_new, e_new)
+    oix[1] = O.build_index(s_rep, e_rep)
+    live = [k for k in list(sets) + [20] if k not in (3, 7)]
+    second = [make(100 + k, live) for k in range(40)]
+    n0 = len(srv.completed)
+    srv.completed.clear()
+    arrivals = serving.poisson_arrivals(400.0, len(second), seed=4)
+    serving.serve_stream(srv, second, arrivals, timeout_s=300)
+    _check(O, oix, second, srv.completed, rids=list(range(n0, n0 + len(second))))
+    assert srv.in_use == 0
+    res.table.close()
