@@ -56,6 +56,8 @@ def parse():
     p.add_argument("--graph", action="store_true",
                    help="small batches: eager plan/offsets/fill vs one captured graph replay")
     p.add_argument("--graph-nq", type=int, default=4096)
+    p.add_argument("--single", action="store_true",
+                   help="microseconds per single-query call (the C-ABI's countOverlaps / searchIdxs path)")
     p.add_argument("--walk-once", action="store_true",
                    help="one launch of each kernel of the 10M x 10M step (plan, offsets, rank count, count walk, "
                         "fill) with the --tuning knobs, for PMC counters")
@@ -643,6 +645,92 @@ def walk_once(a):
     print(json.dumps({"walk_once": True, "queries": nq, "hits": int(off[-1].item()), "tuning": autotune.current()}))
 
 
+def cpu_baseline_single(s, e, qs, qe, per_call):
+    """The reference's own count and search_idxs (its headers, oracle/_ref), one query per
+    call through ctypes, on one host thread; None when that build is absent."""
+    import oracle
+    if not oracle.Reference.available():
+        return None
+    m = oracle.RefMap(oracle.Reference(), s, e)
+    return {"count": round(per_call(lambda x, y: m.count(x, y), qs, qe), 2),
+            "search_idxs": round(per_call(lambda x, y: m.search(x, y, 1), qs, qe), 2)}
+
+
+def bench_single(a):
+    """Latency of single-query calls, the path of the C-ABI's countOverlaps / searchIdxs /
+    searchValues / upperBound / anyOverlaps: one wave on the index's scratch, results in
+    pinned host memory.  Through the C-ABI on 100K intervals (geometric lengths) and through
+    the host-buffer calls on the 10M-interval log-normal index; per call, ctypes included;
+    the reference's own count and search_idxs per call through the same ctypes layer."""
+    import ctypes as C
+    import numpy as np
+    import superintervals_amd as sia
+    import superintervals_amd._native as N
+    L = N.lib
+    out = {"metric": "single-query call latency (microseconds per call, ctypes included)", "unit": "us",
+           "higher_is_better": False, "n_gpus": 1, "data": "synthetic (seeded generators)"}
+    reps = 20000
+
+    def per_call(fn, qs, qe):
+        for k in range(200):
+            fn(int(qs[k]), int(qe[k]))
+        t0 = time.perf_counter()
+        for k in range(reps):
+            fn(int(qs[k]), int(qe[k]))
+        return (time.perf_counter() - t0) / reps * 1e6
+
+    # the C-ABI on a cSuperIntervals of 100K intervals
+    s, e = sia.workload_intervals(1, 100_000, seed=1, sorted=True)
+    qs, qe = sia.workload_queries(1, reps, seed=2)
+    h = L.createSuperIntervals()
+    for i in range(s.size):
+        L.addInterval(h, int(s[i]), int(e[i]), i)
+    L.indexSuperIntervals(h)
+    res = N.cIndexResult()
+
+    def search_idxs(x, y):
+        L.clearIndexResult(C.byref(res))
+        L.searchIdxs(h, x, y, C.byref(res))
+
+    capi = {"countOverlaps": per_call(lambda x, y: L.countOverlaps(h, x, y), qs, qe),
+            "searchIdxs": per_call(search_idxs, qs, qe),
+            "anyOverlaps": per_call(lambda x, y: L.anyOverlaps(h, x, y), qs, qe),
+            "upperBound": per_call(lambda x, y: L.upperBound(h, y), qs, qe)}
+    L.destroyIndexResult(C.byref(res))
+    L.destroySuperIntervals(h)
+    out["capi_100k"] = {k: round(v, 2) for k, v in capi.items()}
+    print("[single] C-ABI, 100K intervals: %s" % out["capi_100k"], file=sys.stderr, flush=True)
+    # host-buffer calls on the 10M-interval index
+    s, e = sia.workload_intervals(2, a.n, seed=1, sorted=True)
+    qs, qe = sia.workload_queries(2, reps, seed=2)
+    ix = sia.DeviceIndex(np.int32).build(s, e)
+    one_s = np.zeros(1, np.int32)
+    one_e = np.zeros(1, np.int32)
+    cnt = np.zeros(1, np.uint32)
+    off = np.zeros(2, np.uint64)
+    buf = np.zeros(1 << 16, np.uint32)
+    tot = C.c_uint64(0)
+
+    def count(x, y):
+        one_s[0], one_e[0] = x, y
+        L.siCountHost(ix.handle, one_s.ctypes.data, one_e.ctypes.data, 1, cnt.ctypes.data)
+
+    def search(x, y):
+        one_s[0], one_e[0] = x, y
+        L.siSearchHost(ix.handle, one_s.ctypes.data, one_e.ctypes.data, 1, 0, off.ctypes.data, buf.ctypes.data,
+                       None, buf.size, C.byref(tot))
+
+    dev = {"count": per_call(count, qs, qe), "search_desc": per_call(search, qs, qe)}
+    out["host_calls_10m"] = {k: round(v, 2) for k, v in dev.items()}
+    print("[single] host-buffer calls, 10M intervals: %s" % out["host_calls_10m"], file=sys.stderr, flush=True)
+    ref = cpu_baseline_single(s, e, qs, qe, per_call)
+    if ref:
+        out["reference_cpu_10m"] = ref
+        print("[single] reference (CPU, 1 thread), 10M intervals: %s" % ref, file=sys.stderr, flush=True)
+    out["value"] = out["host_calls_10m"]["count"]
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -654,6 +742,8 @@ def main():
         return bench_serving(a)
     if a.walk_once:
         return walk_once(a)
+    if a.single:
+        return bench_single(a)
     if a.graph:
         return bench_graph(a)
     if a.skewed:

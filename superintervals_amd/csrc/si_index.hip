@@ -500,6 +500,7 @@ SI_EXPORT void siIndexDestroy(siIndex* ix) {
     if (!ix) return;
     for (siLearned* m : ix->models) si_learned_forget_index(m);
     ix->release();
+    ix->one.release();
     delete ix;
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "si_common.h"
@@ -27,6 +28,31 @@ struct KeyTree {
         d_lv = d_top = nullptr;
         nlv = 0;
         top_size = top_h = 0;
+    }
+};
+
+// Scratch of the single-query calls (the host-buffer forms with one query): a pinned host
+// buffer mapped into the device, which the one-wave kernel writes directly (header + up to
+// cap results), and a stream.  Made on the first such call, kept until the index is
+// destroyed; the last query's results stay in it, so a repeated identical call (a caller
+// that sized its buffer from the first) is answered without a launch.
+struct OneQuery {
+    std::mutex mu;
+    void* h = nullptr;          // pinned host memory: [0] total, [1] spare, then cap uint32 results
+    void* d = nullptr;          // its device address
+    hipStream_t st = nullptr;
+    uint64_t cap = 0;
+    bool last_valid = false;
+    int last_mode = -1;
+    int64_t last_s = 0, last_e = 0;
+    uint64_t last_total = 0;
+    void release() {
+        if (h) (void)hipHostFree(h);
+        if (st) (void)hipStreamDestroy(st);
+        h = d = nullptr;
+        st = nullptr;
+        cap = 0;
+        last_valid = false;
     }
 };
 
@@ -57,6 +83,7 @@ struct siIndex {
     siLearned* learned = nullptr;
     uint64_t generation = 0;
     std::vector<siLearned*> models;   // every model created on this index (siIndexDestroy unlinks them)
+    OneQuery one;                     // single-query scratch (see OneQuery)
     KeyTree ub;   // search tree over the starts (upper_bound)
     KeyTree eb;   // search tree over the sorted ends (rank count)
     int nlev = 0;
@@ -72,6 +99,7 @@ struct siIndex {
         ends32_base = 0;
         n_inverted = 0;
         learned = nullptr;
+        one.last_valid = false;   // the cached last query belongs to the old build
         ++generation;
         ub.release();
         eb.release();

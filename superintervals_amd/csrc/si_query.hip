@@ -1048,6 +1048,101 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix_arg, T
 }
 
 // ============================================================================
+// One query, one wave (the single-query host calls)
+// ============================================================================
+// Last position whose key is <= x through a search tree (UbTree), by one wave, or NONE:
+// the lanes sample every 64th key of the top level (at most 4095 keys), then read the 64
+// keys of the sampled chunk, then at each level below the 64 keys under the entry found
+// (whose first key is that entry).  Keys are sorted, so each ballot is a prefix of lanes.
+template <typename S>
+__device__ __forceinline__ uint32_t wave_tree_ub(const UbTree<S>& t, S x, int lane) {
+    const S* __restrict__ topl = t.lv[t.nlv];
+    const uint32_t m = t.size[t.nlv];
+    uint32_t i = (uint32_t)lane * 64u;
+    uint64_t b = __ballot(i < m && topl[i] <= x);
+    if (!b) return NONE;
+    const uint32_t c = (uint32_t)__popcll(b) - 1u;
+    i = c * 64u + (uint32_t)lane;
+    b = __ballot(i < m && topl[i] <= x);
+    uint32_t j = c * 64u + (uint32_t)__popcll(b) - 1u;
+    for (int l = t.nlv - 1; l >= 0; --l) {
+        i = j * 64u + (uint32_t)lane;
+        b = __ballot(i < t.size[l] && t.lv[l][i] <= x);
+        j = j * 64u + (uint32_t)__popcll(b) - 1u;
+    }
+    return j;
+}
+
+enum OneMode { ONE_UB = 4, ONE_ANY = 5 };
+
+// out (pinned host memory mapped into the device): out[0] = the count (or upper_bound /
+// has_overlaps), then up to cap results as uint32 (positions) or int32 (payload) from
+// out + 2.  The count is exact whatever cap is.
+template <typename S, int MODE>
+__global__ __launch_bounds__(WAVE) void k_one_query(IndexView<S> ix, UbTree<S> ub, UbTree<S> eb, TreeDesc<S> td,
+                                                    const int32_t* __restrict__ vals, uint32_t rank_exact, S s, S e,
+                                                    uint64_t* __restrict__ out, uint64_t cap) {
+    const int lane = lane_id();
+    const uint32_t idx = ix.n ? wave_tree_ub(ub, e, lane) : NONE;
+    if (MODE == ONE_UB) {
+        if (lane == 0) out[0] = idx;
+        return;
+    }
+    if (MODE == ONE_ANY) {   // superintervals.hpp:865-871: the last interval starting at or before e
+        if (lane == 0) out[0] = (idx != NONE && s <= ix.ends[idx]) ? 1u : 0u;
+        return;
+    }
+    if (idx == NONE) {
+        if (lane == 0) out[0] = 0;
+        return;
+    }
+    constexpr S SMIN = (sizeof(S) == 4) ? (S)INT32_MIN : (S)INT64_MIN;
+    if (MODE == W_COUNT && rank_exact && s <= e) {   // the rank form
+        const uint32_t below = (s == SMIN) ? 0u : wave_tree_ub(eb, (S)(s - 1), lane) + 1u;
+        if (lane == 0) out[0] = idx + 1u - below;
+        return;
+    }
+    uint32_t* res = reinterpret_cast<uint32_t*>(out + 2);
+    int32_t* resv = reinterpret_cast<int32_t*>(out + 2);
+    uint64_t acc = 0, found = 0;
+    uint32_t cur = idx, left = 0;
+    bool lead = (MODE == W_VECTOR);
+    if (rank_exact) {
+        // [lower_bound(s), idx]: every position there starts inside the query, all hits
+        const uint32_t lb = (s == SMIN) ? 0u : wave_tree_ub(ub, (S)(s - 1), lane) + 1u;
+        if (lb <= cur) {
+            const uint32_t c = cur - lb + 1u;
+            if (MODE == W_COUNT) {
+                acc += c;
+            } else if (MODE == W_DESC || MODE == W_VALUES) {
+                found += c;
+                const uint64_t w = min((uint64_t)c, cap - acc);
+                for (uint64_t x = lane; x < w; x += WAVE) {
+                    const uint32_t p = cur - (uint32_t)x;
+                    if (MODE == W_VALUES) resv[acc + x] = vals[p];
+                    else res[acc + x] = p;
+                }
+                acc += w;
+            }
+            cur = lb - 1u;   // lb == 0 wraps to NONE
+            if (MODE == W_VECTOR && cur == NONE) {
+                // the leading run reaches position 0: written ascending here (otherwise the
+                // walk below writes it, with the run, once its bottom is found)
+                found += idx + 1u;
+                const uint64_t w = min((uint64_t)(idx + 1u), cap - acc);
+                for (uint64_t x = lane; x < w; x += WAVE) res[acc + x] = (uint32_t)x;
+                acc += w;
+                lead = false;
+            }
+        }
+    }
+    // the rest by the wave walk; left counts down from 0, so -left is what it found
+    wave_walk<S, MODE>(ix, td, lane, s, cur, idx, lead, acc, (MODE == W_COUNT) ? 0 : cap, left, res, vals, resv);
+    const uint64_t total = (MODE == W_COUNT) ? acc : found + (uint64_t)(uint32_t)(0u - left);
+    if (lane == 0) out[0] = total;
+}
+
+// ============================================================================
 // Per-lane kernels over unsorted queries (single-purpose queries)
 // ============================================================================
 template <typename S>
@@ -2248,6 +2343,60 @@ struct DevBuf {
     }
 };
 
+// ---- single queries: one wave, results straight into pinned host memory ------------------
+constexpr uint64_t ONE_CAP = 1u << 16;   // results the single-query scratch holds
+
+// Runs one query (mode W_COUNT / W_DESC / W_VECTOR / W_VALUES / ONE_UB / ONE_ANY) on the
+// index's single-query scratch, made on first use, and waits; *total = its count (or
+// upper_bound / has_overlaps), and up to ONE_CAP results stay in the scratch after the
+// 16-byte header.  The caller holds ix->one.mu.  A repeat of the last call launches nothing.
+template <typename S>
+int one_query_run(siIndex* ix, int mode, S s, S e, uint64_t* total) {
+    OneQuery& o = ix->one;
+    if (!o.h) {
+        HIP_TRY(hipHostMalloc(&o.h, 16 + ONE_CAP * 4, hipHostMallocMapped));
+        HIP_TRY(hipHostGetDevicePointer(&o.d, o.h, 0));
+        HIP_TRY(hipStreamCreateWithFlags(&o.st, hipStreamNonBlocking));
+        o.cap = ONE_CAP;
+        o.last_valid = false;
+    }
+    if (o.last_valid && o.last_mode == mode && o.last_s == (int64_t)s && o.last_e == (int64_t)e) {
+        *total = o.last_total;
+        return SI_OK;
+    }
+    o.last_valid = false;
+    const IndexView<S> v = ix->view<S>();
+    const UbTree<S> ub = ub_tree_of<S>(ix->ub, (const S*)ix->d_starts, ix->n);
+    const UbTree<S> eb = ub_tree_of<S>(ix->eb, (const S*)ix->d_ends_sorted, ix->n);
+    const TreeDesc<S> td = ix->tree<S>();
+    const uint32_t rx = ix->n_inverted == 0 ? 1u : 0u;
+    uint64_t* out = (uint64_t*)o.d;
+    switch (mode) {
+        case W_COUNT: k_one_query<S, W_COUNT><<<1, WAVE, 0, o.st>>>(v, ub, eb, td, ix->d_values, rx, s, e, out, o.cap); break;
+        case W_DESC: k_one_query<S, W_DESC><<<1, WAVE, 0, o.st>>>(v, ub, eb, td, ix->d_values, rx, s, e, out, o.cap); break;
+        case W_VECTOR: k_one_query<S, W_VECTOR><<<1, WAVE, 0, o.st>>>(v, ub, eb, td, ix->d_values, rx, s, e, out, o.cap); break;
+        case W_VALUES: k_one_query<S, W_VALUES><<<1, WAVE, 0, o.st>>>(v, ub, eb, td, ix->d_values, rx, s, e, out, o.cap); break;
+        case ONE_UB: k_one_query<S, ONE_UB><<<1, WAVE, 0, o.st>>>(v, ub, eb, td, ix->d_values, rx, s, e, out, o.cap); break;
+        default: k_one_query<S, ONE_ANY><<<1, WAVE, 0, o.st>>>(v, ub, eb, td, ix->d_values, rx, s, e, out, o.cap); break;
+    }
+    LAUNCH_CHECK();
+    HIP_TRY(hipStreamSynchronize(o.st));
+    *total = ((volatile uint64_t*)o.h)[0];
+    o.last_valid = true;
+    o.last_mode = mode;
+    o.last_s = (int64_t)s;
+    o.last_e = (int64_t)e;
+    o.last_total = *total;
+    return SI_OK;
+}
+
+// One query of a host-buffer call: the coordinates read from qs[0], qe[0].
+int one_query(const siIndex* cix, int mode, const void* qs, const void* qe, uint64_t* total) {
+    siIndex* ix = const_cast<siIndex*>(cix);   // the scratch is the index's mutable state
+    if (ix->cb == 4) return one_query_run<int32_t>(ix, mode, qs ? *(const int32_t*)qs : 0, *(const int32_t*)qe, total);
+    return one_query_run<int64_t>(ix, mode, qs ? *(const int64_t*)qs : 0, *(const int64_t*)qe, total);
+}
+
 int upload_queries(const siIndex* ix, const void* qs, const void* qe, size_t nq, DevBuf& dq) {
     const size_t qb = nq * ix->cb;
     int rc = dq.alloc(2 * qb);
@@ -2263,6 +2412,13 @@ SI_EXPORT int siCountHost(const siIndex* ix, const void* qs, const void* qe, siz
     if (rc) return rc;
     if (nq == 0) return SI_OK;
     if (!qs || !qe || !counts) return si_set_error(SI_ERR_ARG, "null buffer");
+    if (nq == 1) {
+        std::lock_guard<std::mutex> lock(const_cast<siIndex*>(ix)->one.mu);
+        uint64_t c = 0;
+        if ((rc = one_query(ix, W_COUNT, qs, qe, &c))) return rc;
+        counts[0] = (uint32_t)c;
+        return SI_OK;
+    }
     DevBuf dq, dc;
     if ((rc = upload_queries(ix, qs, qe, nq, dq)) || (rc = dc.alloc(nq * 4))) return rc;
     rc = siCount(ix, dq.p, (char*)dq.p + nq * ix->cb, nq, (uint32_t*)dc.p, nullptr);
@@ -2275,6 +2431,14 @@ SI_EXPORT int siUpperBoundHost(const siIndex* ix, const void* xs, size_t n, uint
     int rc = check_ix(ix);
     if (rc) return rc;
     if (n == 0) return SI_OK;
+    if (!xs || !out) return si_set_error(SI_ERR_ARG, "null buffer");
+    if (n == 1) {
+        std::lock_guard<std::mutex> lock(const_cast<siIndex*>(ix)->one.mu);
+        uint64_t u = 0;
+        if ((rc = one_query(ix, ONE_UB, nullptr, xs, &u))) return rc;
+        out[0] = (uint32_t)u;
+        return SI_OK;
+    }
     DevBuf dx, dout;
     if ((rc = dx.alloc(n * ix->cb)) || (rc = dout.alloc(n * 4))) return rc;
     HIP_TRY(hipMemcpy(dx.p, xs, n * ix->cb, hipMemcpyHostToDevice));
@@ -2289,6 +2453,13 @@ SI_EXPORT int siHasOverlapsHost(const siIndex* ix, const void* qs, const void* q
     if (rc) return rc;
     if (nq == 0) return SI_OK;
     if (!qs || !qe || !out) return si_set_error(SI_ERR_ARG, "null buffer");
+    if (nq == 1) {
+        std::lock_guard<std::mutex> lock(const_cast<siIndex*>(ix)->one.mu);
+        uint64_t a = 0;
+        if ((rc = one_query(ix, ONE_ANY, qs, qe, &a))) return rc;
+        out[0] = a ? 1 : 0;
+        return SI_OK;
+    }
     DevBuf dq, dout;
     if ((rc = upload_queries(ix, qs, qe, nq, dq)) || (rc = dout.alloc(nq))) return rc;
     rc = siHasOverlaps(ix, dq.p, (char*)dq.p + nq * ix->cb, nq, (uint8_t*)dout.p, nullptr);
@@ -2322,6 +2493,23 @@ SI_EXPORT int siSearchHost(const siIndex* ix, const void* qs, const void* qe, si
     if (total) *total = 0;
     if (nq == 0) { offsets[0] = 0; return SI_OK; }
     if (!qs || !qe) return si_set_error(SI_ERR_ARG, "null queries");
+    if (values && !ix->d_values) return si_set_error(SI_ERR_ARG, "index was built without values");
+    if (nq == 1) {   // one wave on the index's scratch, unless the result outgrows it
+        std::lock_guard<std::mutex> lock(const_cast<siIndex*>(ix)->one.mu);
+        const int mode = values ? W_VALUES : (order == SI_ORDER_DESC ? W_DESC : W_VECTOR);
+        uint64_t tot = 0;
+        if ((rc = one_query(ix, mode, qs, qe, &tot))) return rc;
+        if (tot <= ONE_CAP) {
+            offsets[0] = 0;
+            offsets[1] = tot;
+            if (total) *total = tot;
+            if (tot > cap) return si_set_error(SI_ERR_CAPACITY, "%llu results exceed capacity %zu", (unsigned long long)tot, cap);
+            if (tot == 0) return SI_OK;
+            if (!values && !idx) return si_set_error(SI_ERR_ARG, "null idx");
+            std::memcpy(values ? (void*)values : (void*)idx, (const char*)ix->one.h + 16, tot * 4);
+            return SI_OK;
+        }
+    }
     const size_t pb = siPlanBytes(ix, nq);
     DevBuf dq, doff, dplan;
     if ((rc = upload_queries(ix, qs, qe, nq, dq)) || (rc = doff.alloc((nq + 1) * 8)) || (rc = dplan.alloc(pb)))

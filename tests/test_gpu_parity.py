@@ -723,6 +723,58 @@ def test_capi_empty_and_unindexed(sia):
     L.destroySuperIntervals(h)
 
 
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_single_query_path(sia, orc, dtype):
+    """One-query host calls run one wave on the index's scratch: every mode against the
+    batched path and the oracle, repeated calls (the capacity probe), a result larger than
+    the scratch (the batched path takes it), an index with start > end intervals."""
+    import superintervals_amd._native as N
+    print(f"[gpu] single-query path dtype={np.dtype(dtype).name}", flush=True)
+    rng = np.random.default_rng(8)
+    n = 300_000
+    s = rng.integers(0, 3_000_000, n)
+    e = s + rng.geometric(1 / 200, n) - 1
+    inv = rng.random(n) < 0.01
+    base = (np.int64(5) << 33) if dtype == np.int64 else 0
+    for with_inv in (False, True):
+        ee = np.where(inv, s - 3, e) if with_inv else e
+        ss_, ee_ = (s + base).astype(dtype), (ee + base).astype(dtype)
+        ix = _dev_index(sia, ss_, ee_, np.arange(n, dtype=np.int32))
+        oix = orc.build_index(ss_, ee_)
+        qs = rng.integers(-10, 3_000_000, 400)
+        qe = qs + rng.integers(0, 5000, qs.size)
+        qe[::13] = qs[::13] - 2                                  # inverted queries
+        qs[0], qe[0] = 0, 3_000_000                               # every interval: > 65536 results
+        qs, qe = (qs + base).astype(dtype), (qe + base).astype(dtype)
+        for q in range(qs.size):
+            a, b = qs[q:q + 1], qe[q:q + 1]
+            assert ix.count(a, b)[0] == orc.count(oix, a[0], b[0])
+            for order, mode in (("desc", 0), ("vector", 1)):
+                off, idx = ix.search(a, b, order=order)   # probe + fill: the second call reuses the first
+                assert np.array_equal(idx.astype(np.int64), orc.search(oix, a[0], b[0], mode)), (q, order)
+            off, vals = ix.search(a, b, values=True)
+            assert np.array_equal(vals, oix["data"][orc.search(oix, a[0], b[0], 0)])
+            ub = int(ix.upper_bound(b)[0])
+            want = orc.upper_bound(oix, b[0])
+            assert (ub == 0xFFFFFFFF and want == -1) or ub == want
+            assert bool(ix.has_overlaps(a, b)[0]) == orc.has_overlaps(oix, a[0], b[0])
+        # capacity probe through the C-ABI-style call: too small, then exact, both right
+        tot = C.c_uint64(0)
+        o2 = np.zeros(2, np.uint64)
+        small = np.zeros(1, np.uint32)
+        a, b = qs[5:6], qe[5:6]
+        want = orc.search(oix, a[0], b[0], 0)
+        if want.size > 1:
+            rc = N.lib.siSearchHost(ix.handle, a.ctypes.data, b.ctypes.data, 1, 0, o2.ctypes.data, small.ctypes.data,
+                                    None, 1, C.byref(tot))
+            assert rc == N.SI_ERR_CAPACITY and tot.value == want.size and o2[1] == want.size
+            big = np.zeros(want.size, np.uint32)
+            assert N.lib.siSearchHost(ix.handle, a.ctypes.data, b.ctypes.data, 1, 0, o2.ctypes.data, big.ctypes.data,
+                                      None, big.size, C.byref(tot)) == 0
+            assert np.array_equal(big.astype(np.int64), want)
+        ix.close()
+
+
 def test_capi_stale_handle_after_clear_and_add(sia):
     """A handle changed after indexSuperIntervals has no device index until it is indexed
     again, even when the new contents have the old size."""
