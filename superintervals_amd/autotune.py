@@ -23,7 +23,7 @@ from . import _native as N
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "superintervals_amd", "tune.json")
 KNOBS = ("tile", "count_wgs", "fill_wgs", "count_rank", "offsets_rank", "lane_steps", "plan_sync", "ub_tree",
-         "plan_skip_low", "heavy_ends32", "range_skip", "ub_learned", "learned_shrink")
+         "plan_skip_low", "heavy_ends32", "range_skip", "ub_learned", "learned_shrink", "long_walk", "long_len")
 CANDIDATE_TILES = (512, 1024, 2048, 4096)
 CANDIDATE_WGS = (1, 0)     # 0 = the occupancy limit
 CANDIDATE_LANE_STEPS = (64, 256, 1024)

@@ -35,3 +35,6 @@ int si_learned_plan_bounds(const siIndex* ix, const void* qe, const void* qs, si
 int si_tune_learned_shrink();
 // The index a model was created on is being destroyed: the model forgets it.
 void si_learned_forget_index(siLearned* m);
+// Build-time tuning of the long list: > 0 the length threshold, 0 chosen by the build from
+// the lengths, < 0 no list.
+int64_t si_tune_long_len();

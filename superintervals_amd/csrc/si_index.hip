@@ -244,6 +244,52 @@ __global__ __launch_bounds__(256) void k_ends32(const int64_t* __restrict__ ends
     if (i < npad) out[i] = (i < n) ? (uint32_t)((uint64_t)ends[i] - (uint64_t)base) : 0u;
 }
 
+// ---- the long list (LongView) ---------------------------------------------------------
+// end - start + 1 > len, in 64-bit arithmetic that cannot overflow (start > end: not long).
+template <typename S>
+__device__ __forceinline__ bool longer_than(S st, S en, int64_t len) {
+    if (en < st) return false;
+    return (uint64_t)en - (uint64_t)st >= (uint64_t)len;   // (end - start + 1) > len
+}
+
+// Lengths of m evenly spaced intervals (for the threshold).
+template <typename S>
+__global__ __launch_bounds__(256) void k_len_sample(const S* __restrict__ st, const S* __restrict__ en, uint32_t n,
+                                                    uint32_t m, int64_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t p = (uint32_t)((uint64_t)i * n / m);
+    const uint64_t d = (uint64_t)en[p] - (uint64_t)st[p];
+    out[i] = (en[p] < st[p]) ? 0 : (d >= (uint64_t)INT64_MAX ? INT64_MAX : (int64_t)d + 1);
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_long_flags(const S* __restrict__ st, const S* __restrict__ en, uint32_t n,
+                                                    int64_t len, uint32_t* __restrict__ flags) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) flags[p] = longer_than(st[p], en[p], len) ? 1u : 0u;
+}
+
+// Entry j = lrank[p] of every long position p; the padding past nlong never hits.
+template <typename S>
+__global__ __launch_bounds__(256) void k_long_fill(const S* __restrict__ en, const uint32_t* __restrict__ en32,
+                                                   const uint32_t* __restrict__ flags, const uint32_t* __restrict__ lrank,
+                                                   uint32_t n, uint32_t nlong, uint32_t npad, uint32_t* __restrict__ pos,
+                                                   S* __restrict__ lends, uint32_t* __restrict__ lends32) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n && flags[p]) {
+        const uint32_t j = lrank[p];
+        pos[j] = p;
+        lends[j] = en[p];
+        if (lends32) lends32[j] = en32[p];
+    }
+    if (p >= nlong && p < npad) {
+        pos[p] = NONE;
+        lends[p] = (sizeof(S) == 4) ? (S)INT32_MIN : (S)INT64_MIN;
+        if (lends32) lends32[p] = 0u;
+    }
+}
+
 // Search tree over n sorted keys: 64-ary levels of first keys up to a top level of at
 // most 4095 keys, which is also stored in Eytzinger order (k_every64, k_eytz).
 template <typename S>
@@ -275,6 +321,86 @@ int build_key_tree(const S* keys, size_t n, KeyTree& t, hipStream_t st) {
     const S* topl = (nl == 0) ? keys : (const S*)t.d_lv + t.lv_off[nl];
     k_eytz<S><<<grid_for((1u << h) - 1u, 256), 256, 0, st>>>(topl, m, h, (S*)t.d_top);
     LAUNCH_CHECK();
+    return SI_OK;
+}
+
+// The long list of a built index: the threshold (the tuning's, or 64 x the median length
+// rounded up to a power of two), the positions and ends of the longer intervals and the max
+// tree over those ends.  Left out (len = 0) for indexes under 2^16 intervals, and when the
+// chosen threshold leaves more than half of the intervals long.
+template <typename S>
+int build_long_list(siIndex* ix, hipStream_t st) {
+    const uint32_t n = (uint32_t)ix->n;
+    int64_t len = si_tune_long_len();
+    const bool automatic = len == 0;
+    if (len < 0 || (automatic && n < (1u << 16))) return SI_OK;
+    const S* d_st = (const S*)ix->d_starts;
+    const S* d_en = (const S*)ix->d_ends;
+    if (automatic) {
+        constexpr uint32_t M = 4096;
+        int64_t* d_s = nullptr;
+        HIP_TRY(hipMalloc(&d_s, M * sizeof(int64_t)));
+        std::vector<int64_t> h(M);
+        k_len_sample<S><<<grid_for(M, 256), 256, 0, st>>>(d_st, d_en, n, M, d_s);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d_s, M * sizeof(int64_t), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        hipFree(d_s);
+        if (e != hipSuccess) return si_set_error(SI_ERR_HIP, "long list lengths: %s", hipGetErrorString(e));
+        std::nth_element(h.begin(), h.begin() + M / 2, h.end());
+        const int64_t med = std::max<int64_t>(h[M / 2], 1);
+        len = 1;
+        while (len < 64 * med && len < ((int64_t)1 << 62)) len <<= 1;
+    }
+    LongList& L = ix->lng;
+    uint32_t* flags = nullptr;
+    uint32_t* ws = nullptr;
+    HIP_TRY(hipMalloc(&flags, (size_t)n * 4));
+    HIP_TRY(hipMalloc(&L.d_lrank, ((size_t)n + 1) * 4));
+    HIP_TRY(hipMalloc(&ws, scan_ws_items(n) * 4));
+    k_long_flags<S><<<grid_for(n, 256), 256, 0, st>>>(d_st, d_en, n, len, flags);
+    LAUNCH_CHECK();
+    int rc = device_excl_scan<uint32_t, uint32_t>(flags, n, L.d_lrank, L.d_lrank + n, ws, st);
+    uint32_t nlong = 0;
+    if (rc == SI_OK) {
+        hipError_t e = hipMemcpyAsync(&nlong, L.d_lrank + n, 4, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = si_set_error(SI_ERR_HIP, "long list count: %s", hipGetErrorString(e));
+    }
+    hipFree(ws);
+    if (rc != SI_OK || nlong == 0 || (automatic && nlong > n / 2)) {
+        hipFree(flags);
+        L.release();
+        return rc;
+    }
+    const uint32_t npad = (nlong + 63u) & ~63u;
+    HIP_TRY(hipMalloc(&L.d_pos, (size_t)npad * 4));
+    HIP_TRY(hipMalloc(&L.d_ends, (size_t)npad * sizeof(S)));
+    if (ix->d_ends32) HIP_TRY(hipMalloc(&L.d_ends32, (size_t)npad * 4));
+    k_long_fill<S><<<grid_for(std::max(n, npad), 256), 256, 0, st>>>(d_en, ix->d_ends32, flags, L.d_lrank, n, nlong,
+                                                                     npad, L.d_pos, (S*)L.d_ends, L.d_ends32);
+    LAUNCH_CHECK();
+    // max tree over the long ends, as over the index's ends
+    std::vector<uint32_t> sizes{nlong};
+    while (sizes.back() > (uint32_t)FAN && (int)sizes.size() < MAX_LEVELS)
+        sizes.push_back((sizes.back() + FAN - 1) / FAN);
+    L.nlev = (int)sizes.size();
+    size_t tot = 0;
+    for (int l = 1; l < L.nlev; ++l) { L.lvl_off[l] = tot; L.lvl_size[l] = sizes[l]; tot += sizes[l]; }
+    L.lvl_size[0] = nlong;
+    if (tot) {
+        HIP_TRY(hipMalloc(&L.d_tree, tot * sizeof(S)));
+        for (int l = 1; l < L.nlev; ++l) {
+            const S* in = (l == 1) ? (const S*)L.d_ends : (const S*)L.d_tree + L.lvl_off[l - 1];
+            k_tree_level<S><<<grid_for((size_t)sizes[l] * FAN, 256), 256, 0, st>>>(in, sizes[l - 1],
+                                                                                   (S*)L.d_tree + L.lvl_off[l], sizes[l]);
+            LAUNCH_CHECK();
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    hipFree(flags);
+    L.len = len;
+    L.n = nlong;
     return SI_OK;
 }
 
@@ -466,6 +592,10 @@ int build_impl(siIndex* ix, const void* starts, const void* ends, const int32_t*
                                                            ix->d_ends32);
             LAUNCH_CHECK();
         }
+    }
+    {
+        const int rc = build_long_list<S>(ix, st);
+        if (rc != SI_OK) return rc;
     }
     HIP_TRY(hipStreamSynchronize(st));
     if (!on_device) { hipFree(in_s); hipFree(in_e); if (in_v) hipFree(in_v); }

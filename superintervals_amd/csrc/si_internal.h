@@ -56,6 +56,28 @@ struct OneQuery {
     }
 };
 
+// The long list of an index (LongView): device arrays and the max tree over its ends.
+struct LongList {
+    int64_t len = 0;              // threshold; 0 = no list
+    uint32_t n = 0;
+    uint32_t* d_pos = nullptr;    // n, padded to 64 entries
+    void* d_ends = nullptr;       // n, padded with the minimum
+    uint32_t* d_ends32 = nullptr; // int64 indexes with a 32-bit copy of the ends
+    uint32_t* d_lrank = nullptr;  // index size + 1
+    void* d_tree = nullptr;       // levels 1.. of the max tree over d_ends
+    int nlev = 0;
+    size_t lvl_off[MAX_LEVELS] = {0};
+    uint32_t lvl_size[MAX_LEVELS] = {0};
+    void release() {
+        hipFree(d_pos); hipFree(d_ends); hipFree(d_ends32); hipFree(d_lrank); hipFree(d_tree);
+        d_pos = d_ends32 = d_lrank = nullptr;
+        d_ends = d_tree = nullptr;
+        len = 0;
+        n = 0;
+        nlev = 0;
+    }
+};
+
 // ============================================================================
 // siIndex
 // ============================================================================
@@ -84,6 +106,7 @@ struct siIndex {
     uint64_t generation = 0;
     std::vector<siLearned*> models;   // every model created on this index (siIndexDestroy unlinks them)
     OneQuery one;                     // single-query scratch (see OneQuery)
+    LongList lng;                     // intervals longer than a threshold (LongView)
     KeyTree ub;   // search tree over the starts (upper_bound)
     KeyTree eb;   // search tree over the sorted ends (rank count)
     int nlev = 0;
@@ -100,6 +123,7 @@ struct siIndex {
         n_inverted = 0;
         learned = nullptr;
         one.last_valid = false;   // the cached last query belongs to the old build
+        lng.release();
         ++generation;
         ub.release();
         eb.release();
@@ -121,6 +145,27 @@ struct siIndex {
         v.ends32_base = ends32_base;
         v.n = (uint32_t)n;
         v.npad = (uint32_t)((n + 63) & ~(size_t)63);
+        return v;
+    }
+
+    template <typename S>
+    LongView<S> long_view() const {
+        LongView<S> v;
+        v.pos = lng.d_pos;
+        v.ends = (const S*)lng.d_ends;
+        v.ends32 = lng.d_ends32;
+        v.ends32_base = ends32_base;
+        v.lrank = lng.d_lrank;
+        v.len = lng.len;
+        v.n = lng.n;
+        v.td.nlev = lng.nlev;
+        v.td.lvl[0] = (const S*)lng.d_ends;
+        v.td.size[0] = lng.n;
+        for (int l = 1; l < lng.nlev; ++l) {
+            v.td.lvl[l] = (const S*)lng.d_tree + lng.lvl_off[l];
+            v.td.size[l] = lng.lvl_size[l];
+        }
+        for (int l = (lng.nlev > 0 ? lng.nlev : 1); l < MAX_LEVELS; ++l) { v.td.lvl[l] = nullptr; v.td.size[l] = 0; }
         return v;
     }
 

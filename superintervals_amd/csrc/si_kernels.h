@@ -44,6 +44,24 @@ struct IndexView {
     uint32_t npad;   // allocated length of starts/ends/branch (multiple of 64)
 };
 
+// Intervals longer than len (the long list), in position order: their positions, their ends
+// (and ends - ends32_base as uint32 where the index keeps a 32-bit copy), a max tree over
+// those ends (td.lvl[0] = ends), and lrank[p] = the number of them before position p.  The
+// wave walk takes every hit below the first position starting at or after s - len from this
+// list: an interval starting before s - len reaches s only if it is longer than len.
+// len == 0: no list.
+template <typename S>
+struct LongView {
+    const uint32_t* __restrict__ pos;
+    const S* __restrict__ ends;
+    const uint32_t* __restrict__ ends32;
+    int64_t ends32_base;
+    const uint32_t* __restrict__ lrank;
+    TreeDesc<S> td;
+    int64_t len;
+    uint32_t n;
+};
+
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 // ---- upper_bound ------------------------------------------------------------

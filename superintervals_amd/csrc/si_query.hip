@@ -285,6 +285,10 @@ __global__ __launch_bounds__(Q_THREADS) void k_count_rank(IndexView<S> ix, const
 //                trained on the index (siLearnedTrain), when it has one; 0: the tree
 //   learned_shrink  test only: the learned model's leaf windows narrowed by this many
 //                positions per side, which forces its whole-leaf fallback (0 normally)
+//   long_walk    1: the wave walk takes the hits below s - len from the index's long list
+//                when it has one; 0: it walks the index's arrays all the way
+//   long_len     read by the build: > 0 the long list's length threshold, 0 chosen from the
+//                lengths (64 x the median, a power of two), < 0 no long list
 struct Tuning {
     uint32_t tile = 1024;
     int count_wgs = 0;
@@ -299,8 +303,18 @@ struct Tuning {
     int range_skip = 1;
     int ub_learned = 0;
     int learned_shrink = 0;
+    int long_walk = 1;
+    int64_t long_len = 0;
 };
 Tuning g_tune;
+
+// The index's long list as the walks see it (none when the knob long_walk is 0).
+template <typename S>
+LongView<S> long_view_of(const siIndex* ix) {
+    LongView<S> v = ix->long_view<S>();
+    if (!g_tune.long_walk) v.len = 0;
+    return v;
+}
 
 // The run [lower_bound(s), upper_bound(e)] holds only hits when every interval has
 // start <= end (each starts at or after s, so ends at or after s).
@@ -857,17 +871,48 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
 #endif
 constexpr uint32_t WAVE_LINES = SI_WAVE_LINES;   // lines with hits fetched per round trip
 
+// Last position whose key is <= x through a search tree (UbTree), by one wave, or NONE:
+// the lanes sample every 64th key of the top level (at most 4095 keys), then read the 64
+// keys of the sampled chunk, then at each level below the 64 keys under the entry found
+// (whose first key is that entry).  Keys are sorted, so each ballot is a prefix of lanes.
 template <typename S>
-__device__ __forceinline__ S wave_end(const IndexView<S>& ix, uint32_t i) {
-    if (sizeof(S) == 8 && ix.ends32) return (S)((int64_t)ix.ends32[i] + ix.ends32_base);   // int64 ends within 2^32
-    return ix.ends[i];
+__device__ __forceinline__ uint32_t wave_tree_ub(const UbTree<S>& t, S x, int lane) {
+    const S* __restrict__ topl = t.lv[t.nlv];
+    const uint32_t m = t.size[t.nlv];
+    uint32_t i = (uint32_t)lane * 64u;
+    uint64_t b = __ballot(i < m && topl[i] <= x);
+    if (!b) return NONE;
+    const uint32_t c = (uint32_t)__popcll(b) - 1u;
+    i = c * 64u + (uint32_t)lane;
+    b = __ballot(i < m && topl[i] <= x);
+    uint32_t j = c * 64u + (uint32_t)__popcll(b) - 1u;
+    for (int l = t.nlv - 1; l >= 0; --l) {
+        i = j * 64u + (uint32_t)lane;
+        b = __ballot(i < t.size[l] && t.lv[l][i] <= x);
+        j = j * 64u + (uint32_t)__popcll(b) - 1u;
+    }
+    return j;
 }
 
-// Highest line l < L0 whose level-1 maximum is >= s, or NONE: the entries [g, pos) of a
-// level's group by one ballot, else the level above over [0, g); once found, down to the
-// highest child >= s at each level (a parent's maximum is its children's).
+// An entry of an ends array, through its 32-bit copy where one is kept (int64 ends within
+// 2^32 of their minimum).
 template <typename S>
-__device__ __forceinline__ uint32_t prev_line(const TreeDesc<S>& td, uint32_t L0, S s, int lane) {
+__device__ __forceinline__ S end_at(const S* __restrict__ ends, const uint32_t* __restrict__ ends32, int64_t base,
+                                    uint32_t i) {
+    if (sizeof(S) == 8 && ends32) return (S)((int64_t)ends32[i] + base);
+    return ends[i];
+}
+
+template <typename S>
+__device__ __forceinline__ S wave_end(const IndexView<S>& ix, uint32_t i) {
+    return end_at(ix.ends, ix.ends32, ix.ends32_base, i);
+}
+
+// Highest line l in [lo_line, L0) whose level-1 maximum is >= s, or NONE: the entries
+// [g, pos) of a level's group by one ballot, else the level above over [0, g); once found,
+// down to the highest child >= s at each level (a parent's maximum is its children's).
+template <typename S>
+__device__ __forceinline__ uint32_t prev_line(const TreeDesc<S>& td, uint32_t L0, uint32_t lo_line, S s, int lane) {
     constexpr S SMIN = (sizeof(S) == 4) ? (S)INT32_MIN : (S)INT64_MIN;
     uint32_t pos = L0;   // exclusive bound at level l
     for (int l = 1; l < td.nlev; ++l) {
@@ -885,8 +930,10 @@ __device__ __forceinline__ uint32_t prev_line(const TreeDesc<S>& td, uint32_t L0
                 if (!mm) return NONE;   // cannot happen: the parent's maximum is a child's
                 node = node * 64u + 63u - (uint32_t)__builtin_clzll(mm);
             }
-            return node;
+            return node >= lo_line ? node : NONE;
         }
+        // the lines left are [0, g * 64^(l-1)): none at or above lo_line, or the level above
+        if (((uint64_t)g << (6 * (l - 1))) <= (uint64_t)lo_line) return NONE;
         pos = g / 64u;   // level l + 1 entries [0, g / 64) cover level l entries [0, g)
     }
     return NONE;
@@ -899,15 +946,88 @@ __device__ __forceinline__ uint64_t group_mask(const TreeDesc<S>& td, uint32_t g
     return __ballot(i < td.size[1] && td.lvl[1][i] >= s);
 }
 
+// The hits among entries [lo, cur] of an ends array with max tree td (td.lvl[0] = the ends),
+// highest first: one 64-entry line per step, the lines with a maximum >= s only.  The output
+// value of entry i is posmap[i] (the long list), or i itself.  Count and fill modes as
+// wave_walk.
+template <typename S, int MODE>
+__device__ __forceinline__ void guided_walk(const S* __restrict__ ends, const uint32_t* __restrict__ ends32,
+                                            int64_t base32, const uint32_t* __restrict__ posmap,
+                                            const TreeDesc<S>& td, int lane, S s, uint32_t cur, uint32_t lo,
+                                            uint64_t& acc, uint64_t oend, uint32_t& left,
+                                            uint32_t* __restrict__ out_idx, const int32_t* __restrict__ vals,
+                                            int32_t* __restrict__ out_val) {
+    const uint64_t above = (lane == 63) ? 0ull : (~0ull << (lane + 1));   // lanes above this one
+    auto take = [&](uint32_t b0, uint32_t kc, uint32_t klo, S v) {
+        const bool hit = ((uint32_t)lane <= kc) && ((uint32_t)lane >= klo) && (v >= s);
+        const uint64_t m = __ballot(hit);
+        const uint32_t nh = (uint32_t)__popcll(m);
+        if (MODE == W_COUNT) {
+            acc += nh;
+        } else {
+            left -= nh;
+            const uint64_t r = (uint64_t)__popcll(m & above);   // hits above this lane
+            if (hit && acc + r < oend) {
+                const uint32_t p = posmap ? posmap[b0 + lane] : b0 + (uint32_t)lane;
+                if (MODE == W_VALUES) out_val[acc + r] = vals[p];
+                else out_idx[acc + r] = p;
+            }
+            acc = min(acc + nh, oend);
+        }
+    };
+    if (cur == NONE || cur < lo) return;
+    if (td.nlev < 2) {   // one line only (no level above the ends)
+        take(0u, cur, lo, end_at(ends, ends32, base32, (uint32_t)lane));
+        return;
+    }
+    const uint32_t L0 = cur >> 6;        // the first line, walked at and below cur only
+    const uint32_t lo_line = lo >> 6;    // the last line, walked at and above lo only
+    uint32_t g1 = L0 & ~63u;             // group of 64 lines in hand
+    const uint32_t sh = L0 - g1;
+    uint64_t rem = group_mask(td, g1, s, lane) & ((sh == 63u) ? ~0ull : ((2ull << sh) - 1ull));
+    while (true) {
+        if (lo_line >= g1) rem &= ~((1ull << (lo_line - g1)) - 1ull);   // lines below lo_line are out
+        if (rem == 0) {
+            if (g1 <= lo_line) return;
+            const uint32_t l = prev_line(td, g1, lo_line, s, lane);
+            if (l == NONE) return;
+            g1 = l & ~63u;
+            rem = group_mask(td, g1, s, lane) & ((2ull << (l - g1)) - 1ull);   // lines of the group up to l
+            continue;
+        }
+        // the next WAVE_LINES lines with hits, highest first (NONE past the group's last)
+        uint32_t ln[WAVE_LINES];
+        S v[WAVE_LINES];
+#pragma unroll
+        for (uint32_t j = 0; j < WAVE_LINES; ++j) {
+            ln[j] = rem ? g1 + 63u - (uint32_t)__builtin_clzll(rem) : NONE;
+            rem = rem ? (rem & ~(1ull << (ln[j] - g1))) : 0ull;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < WAVE_LINES; ++j) {
+            const uint32_t b0 = (ln[j] == NONE) ? 0u : ln[j] * 64u;   // absent lines read line 0, unused
+            v[j] = end_at(ends, ends32, base32, b0 + (uint32_t)lane);
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < WAVE_LINES; ++j) {
+            if (ln[j] == NONE) continue;
+            const uint32_t b0 = ln[j] * 64u;
+            take(b0, (ln[j] == L0) ? cur - b0 : 63u, (ln[j] == lo_line) ? lo - b0 : 0u, v[j]);
+        }
+    }
+}
+
 // Walks the hits at and below cur (see above).  acc: the count so far (count mode) or the
 // next output offset (fill modes, writes below oend only); left: in fill modes the hits
-// still expected, counted down.
+// still expected, counted down.  With a long list (lv.len > 0) the positions below P0 = the
+// first position starting at or after s - lv.len hold no hit but a long interval, so the
+// walk reads the index's arrays down to P0 only and takes the rest from the long list,
+// whose entries all lie below P0: the output order is unchanged.
 template <typename S, int MODE>
-__device__ __forceinline__ void wave_walk(const IndexView<S>& ix, const TreeDesc<S>& td, int lane, S s, uint32_t cur,
-                                          uint32_t top, bool lead, uint64_t& acc, uint64_t oend, uint32_t& left,
-                                          uint32_t* __restrict__ out_idx, const int32_t* __restrict__ vals,
-                                          int32_t* __restrict__ out_val) {
-    const uint64_t above = (lane == 63) ? 0ull : (~0ull << (lane + 1));   // lanes above this one
+__device__ __forceinline__ void wave_walk(const IndexView<S>& ix, const TreeDesc<S>& td, const UbTree<S>& ub,
+                                          const LongView<S>& lv, int lane, S s, uint32_t cur, uint32_t top, bool lead,
+                                          uint64_t& acc, uint64_t oend, uint32_t& left, uint32_t* __restrict__ out_idx,
+                                          const int32_t* __restrict__ vals, int32_t* __restrict__ out_val) {
     // leading run (vector order): consecutive lines while every position down from cur is a hit
     while (MODE == W_VECTOR && lead && cur != NONE) {
         const uint32_t b0 = cur & ~63u;
@@ -932,76 +1052,27 @@ __device__ __forceinline__ void wave_walk(const IndexView<S>& ix, const TreeDesc
         }
     }
     if (cur == NONE) return;
-    if (td.nlev < 2) {
-        // one line only (no level above the ends): its hits at and below cur
-        const bool hit = ((uint32_t)lane <= cur) && (wave_end(ix, (uint32_t)lane) >= s);
-        const uint64_t m = __ballot(hit);
-        const uint32_t nh = (uint32_t)__popcll(m);
-        if (MODE == W_COUNT) {
-            acc += nh;
-        } else {
-            left -= nh;
-            const uint64_t r = (uint64_t)__popcll(m & above);
-            if (hit && acc + r < oend) {
-                if (MODE == W_VALUES) out_val[acc + r] = vals[lane];
-                else out_idx[acc + r] = (uint32_t)lane;
-            }
-            acc = min(acc + nh, oend);
-        }
-        return;
+    uint32_t P0 = 0;
+    if (lv.len > 0) {
+        constexpr S SMIN = (sizeof(S) == 4) ? (S)INT32_MIN : (S)INT64_MIN;
+        // x = s - len - 1 (upper_bound(x) + 1 = the first position starting at >= s - len)
+        if ((int64_t)s > (int64_t)SMIN + lv.len) P0 = wave_tree_ub(ub, (S)((int64_t)s - lv.len - 1), lane) + 1u;
     }
-    const uint32_t L0 = cur >> 6;        // the first line, walked at and below cur only
-    uint32_t g1 = L0 & ~63u;             // group of 64 lines in hand
-    const uint32_t sh = L0 - g1;
-    uint64_t rem = group_mask(td, g1, s, lane) & ((sh == 63u) ? ~0ull : ((2ull << sh) - 1ull));
-    while (true) {
-        if (rem == 0) {
-            const uint32_t l = prev_line(td, g1, s, lane);
-            if (l == NONE) return;
-            g1 = l & ~63u;
-            rem = group_mask(td, g1, s, lane) & ((2ull << (l - g1)) - 1ull);   // lines of the group up to l
-            continue;
-        }
-        // the next WAVE_LINES lines with hits, highest first (NONE past the group's last)
-        uint32_t ln[WAVE_LINES];
-        S v[WAVE_LINES];
-#pragma unroll
-        for (uint32_t j = 0; j < WAVE_LINES; ++j) {
-            ln[j] = rem ? g1 + 63u - (uint32_t)__builtin_clzll(rem) : NONE;
-            rem = rem ? (rem & ~(1ull << (ln[j] - g1))) : 0ull;
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < WAVE_LINES; ++j) {
-            const uint32_t b0 = (ln[j] == NONE) ? 0u : ln[j] * 64u;   // absent lines read line 0, unused
-            v[j] = wave_end(ix, b0 + (uint32_t)lane);
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < WAVE_LINES; ++j) {
-            if (ln[j] == NONE) continue;
-            const uint32_t b0 = ln[j] * 64u;
-            const uint32_t kc = (ln[j] == L0) ? cur - b0 : 63u;
-            const bool hit = ((uint32_t)lane <= kc) && (v[j] >= s);
-            const uint64_t m = __ballot(hit);
-            const uint32_t nh = (uint32_t)__popcll(m);
-            if (MODE == W_COUNT) {
-                acc += nh;
-            } else {
-                left -= nh;
-                const uint64_t r = (uint64_t)__popcll(m & above);   // hits above this lane
-                if (hit && acc + r < oend) {
-                    if (MODE == W_VALUES) out_val[acc + r] = vals[b0 + lane];
-                    else out_idx[acc + r] = b0 + (uint32_t)lane;
-                }
-                acc = min(acc + nh, oend);
-            }
-        }
+    guided_walk<S, MODE>(ix.ends, ix.ends32, ix.ends32_base, nullptr, td, lane, s, cur, P0, acc, oend, left, out_idx,
+                         vals, out_val);
+    if (P0 > 0) {
+        const uint32_t J = lv.lrank[min(P0, cur + 1u)];   // long entries below that position
+        if (J > 0)
+            guided_walk<S, MODE>(lv.ends, lv.ends32, lv.ends32_base, lv.pos, lv.td, lane, s, J - 1u, 0u, acc, oend,
+                                 left, out_idx, vals, out_val);
     }
 }
 
 // Continuation of walks the lane walk parked (over its step budget): one wave per parked
 // query, taken from a global counter.
 template <typename S, int MODE, bool MULTI = false>
-__global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix_arg, TreeDesc<S> td_arg, PlanView<S> pv,
+__global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix_arg, TreeDesc<S> td_arg, UbTree<S> ub_arg,
+                                                          LongView<S> lv, PlanView<S> pv,
                                                           uint32_t* __restrict__ counts,
                                                           uint64_t* __restrict__ counts64,
                                                           const uint64_t* __restrict__ offsets,
@@ -1011,6 +1082,7 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix_arg, T
                                                           const SetView<S>* __restrict__ sets) {
     IndexView<S> ix = ix_arg;
     TreeDesc<S> td = td_arg;
+    UbTree<S> ub = ub_arg;
     const int32_t* __restrict__ vals = vals_arg;
     const int lane = lane_id();
     const uint32_t nheavy = __builtin_amdgcn_readfirstlane(pv.counter[1]);
@@ -1026,6 +1098,7 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix_arg, T
             const uint32_t g = __builtin_amdgcn_readfirstlane(pv.set[k]);
             ix = sets[g].v;
             td = sets[g].td;
+            ub = sets[g].ub;
             vals = sets[g].values;
         }
         const uint32_t q = __builtin_amdgcn_readfirstlane(pv.q[k]);
@@ -1036,7 +1109,7 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix_arg, T
         uint32_t left = __builtin_amdgcn_readfirstlane(he.left);   // fill modes: hits still expected
         uint64_t acc = he.acc;
         const uint64_t oend = (MODE != W_COUNT) ? min(offsets[q + 1], cap) : 0;
-        wave_walk<S, MODE>(ix, td, lane, s, cur, top, lead, acc, oend, left, out_idx, vals, out_val);
+        wave_walk<S, MODE>(ix, td, ub, lv, lane, s, cur, top, lead, acc, oend, left, out_idx, vals, out_val);
         if (MODE == W_COUNT) {
             // every lane stores the same value (no lane-dependent branch in the loop)
             if (counts) counts[q] = (uint32_t)acc;
@@ -1050,29 +1123,6 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix_arg, T
 // ============================================================================
 // One query, one wave (the single-query host calls)
 // ============================================================================
-// Last position whose key is <= x through a search tree (UbTree), by one wave, or NONE:
-// the lanes sample every 64th key of the top level (at most 4095 keys), then read the 64
-// keys of the sampled chunk, then at each level below the 64 keys under the entry found
-// (whose first key is that entry).  Keys are sorted, so each ballot is a prefix of lanes.
-template <typename S>
-__device__ __forceinline__ uint32_t wave_tree_ub(const UbTree<S>& t, S x, int lane) {
-    const S* __restrict__ topl = t.lv[t.nlv];
-    const uint32_t m = t.size[t.nlv];
-    uint32_t i = (uint32_t)lane * 64u;
-    uint64_t b = __ballot(i < m && topl[i] <= x);
-    if (!b) return NONE;
-    const uint32_t c = (uint32_t)__popcll(b) - 1u;
-    i = c * 64u + (uint32_t)lane;
-    b = __ballot(i < m && topl[i] <= x);
-    uint32_t j = c * 64u + (uint32_t)__popcll(b) - 1u;
-    for (int l = t.nlv - 1; l >= 0; --l) {
-        i = j * 64u + (uint32_t)lane;
-        b = __ballot(i < t.size[l] && t.lv[l][i] <= x);
-        j = j * 64u + (uint32_t)__popcll(b) - 1u;
-    }
-    return j;
-}
-
 enum OneMode { ONE_UB = 4, ONE_ANY = 5 };
 
 // out (pinned host memory mapped into the device): out[0] = the count (or upper_bound /
@@ -1080,6 +1130,7 @@ enum OneMode { ONE_UB = 4, ONE_ANY = 5 };
 // out + 2.  The count is exact whatever cap is.
 template <typename S, int MODE>
 __global__ __launch_bounds__(WAVE) void k_one_query(IndexView<S> ix, UbTree<S> ub, UbTree<S> eb, TreeDesc<S> td,
+                                                    LongView<S> lv,
                                                     const int32_t* __restrict__ vals, uint32_t rank_exact, S s, S e,
                                                     uint64_t* __restrict__ out, uint64_t cap) {
     const int lane = lane_id();
@@ -1137,7 +1188,7 @@ __global__ __launch_bounds__(WAVE) void k_one_query(IndexView<S> ix, UbTree<S> u
         }
     }
     // the rest by the wave walk; left counts down from 0, so -left is what it found
-    wave_walk<S, MODE>(ix, td, lane, s, cur, idx, lead, acc, (MODE == W_COUNT) ? 0 : cap, left, res, vals, resv);
+    wave_walk<S, MODE>(ix, td, ub, lv, lane, s, cur, idx, lead, acc, (MODE == W_COUNT) ? 0 : cap, left, res, vals, resv);
     const uint64_t total = (MODE == W_COUNT) ? acc : found + (uint64_t)(uint32_t)(0u - left);
     if (lane == 0) out[0] = total;
 }
@@ -1463,10 +1514,11 @@ int launch_walk(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint
         cus = std::max(cus, 1);
     }
     IndexView<S> hv = ix->view<S>();
-    if (!g_tune.heavy_ends32) hv.ends32 = nullptr;
-    k_walk_heavy<S, MODE><<<(unsigned)cus * 8u, Q_THREADS, 0, st>>>(hv, ix->tree<S>(), pv, counts, counts64,
-                                                                   offsets, out_idx, ix->d_values, out_val, cap,
-                                                                   nullptr);
+    LongView<S> lv = long_view_of<S>(ix);
+    if (!g_tune.heavy_ends32) { hv.ends32 = nullptr; lv.ends32 = nullptr; }
+    k_walk_heavy<S, MODE><<<(unsigned)cus * 8u, Q_THREADS, 0, st>>>(
+        hv, ix->tree<S>(), ub_tree_of<S>(ix->ub, (const S*)ix->d_starts, ix->n), lv, pv, counts, counts64, offsets,
+        out_idx, ix->d_values, out_val, cap, nullptr);
     LAUNCH_CHECK();
     return SI_OK;
 }
@@ -1502,6 +1554,7 @@ struct ScopedPlan {
 // Tuning
 // ============================================================================
 int si_tune_learned_shrink() { return g_tune.learned_shrink; }
+int64_t si_tune_long_len() { return g_tune.long_len; }
 
 SI_EXPORT int siTuneSet(const char* name, int64_t value) {
     if (!name) return si_set_error(SI_ERR_ARG, "null tuning name");
@@ -1527,6 +1580,11 @@ SI_EXPORT int siTuneSet(const char* name, int64_t value) {
     } else if (k == "heavy_ends32") {
         if (value != 0 && value != 1) return si_set_error(SI_ERR_ARG, "heavy_ends32 must be 0 or 1");
         g_tune.heavy_ends32 = (int)value;
+    } else if (k == "long_walk") {
+        if (value != 0 && value != 1) return si_set_error(SI_ERR_ARG, "long_walk must be 0 or 1");
+        g_tune.long_walk = (int)value;
+    } else if (k == "long_len") {
+        g_tune.long_len = value;
     } else if (k == "ub_learned") {
         if (value != 0 && value != 1) return si_set_error(SI_ERR_ARG, "ub_learned must be 0 or 1");
         g_tune.ub_learned = (int)value;
@@ -1561,6 +1619,8 @@ SI_EXPORT int64_t siTuneGet(const char* name) {
     if (k == "heavy_ends32") return g_tune.heavy_ends32;
     if (k == "range_skip") return g_tune.range_skip;
     if (k == "ub_learned") return g_tune.ub_learned;
+    if (k == "long_walk") return g_tune.long_walk;
+    if (k == "long_len") return g_tune.long_len;
     if (k == "learned_shrink") return g_tune.learned_shrink;
     if (k == "count_rank") return g_tune.count_rank;
     if (k == "offsets_rank") return g_tune.offsets_rank;
@@ -2144,9 +2204,10 @@ int launch_walk_multi(const siSetTable* tb, void* plan, size_t nq, const uint64_
         IndexView<int32_t>{}, pv, (uint32_t)nq, tile, NONE - 1u, g_tune.lane_steps, true, pv.counter, nullptr,
         nullptr, offsets, out_idx, nullptr, nullptr, cap, sets, mt);
     LAUNCH_CHECK();
+    LongView<int32_t> none{};   // the sets' walks read their own arrays only
     k_walk_heavy<int32_t, MODE, true><<<(unsigned)cus * 8u, Q_THREADS, 0, st>>>(
-        IndexView<int32_t>{}, TreeDesc<int32_t>{}, pv, nullptr, nullptr, offsets, out_idx, nullptr, nullptr, cap,
-        sets);
+        IndexView<int32_t>{}, TreeDesc<int32_t>{}, UbTree<int32_t>{}, none, pv, nullptr, nullptr, offsets, out_idx,
+        nullptr, nullptr, cap, sets);
     LAUNCH_CHECK();
     return SI_OK;
 }
@@ -2369,15 +2430,16 @@ int one_query_run(siIndex* ix, int mode, S s, S e, uint64_t* total) {
     const UbTree<S> ub = ub_tree_of<S>(ix->ub, (const S*)ix->d_starts, ix->n);
     const UbTree<S> eb = ub_tree_of<S>(ix->eb, (const S*)ix->d_ends_sorted, ix->n);
     const TreeDesc<S> td = ix->tree<S>();
+    const LongView<S> lv = long_view_of<S>(ix);
     const uint32_t rx = ix->n_inverted == 0 ? 1u : 0u;
     uint64_t* out = (uint64_t*)o.d;
     switch (mode) {
-        case W_COUNT: k_one_query<S, W_COUNT><<<1, WAVE, 0, o.st>>>(v, ub, eb, td, ix->d_values, rx, s, e, out, o.cap); break;
-        case W_DESC: k_one_query<S, W_DESC><<<1, WAVE, 0, o.st>>>(v, ub, eb, td, ix->d_values, rx, s, e, out, o.cap); break;
-        case W_VECTOR: k_one_query<S, W_VECTOR><<<1, WAVE, 0, o.st>>>(v, ub, eb, td, ix->d_values, rx, s, e, out, o.cap); break;
-        case W_VALUES: k_one_query<S, W_VALUES><<<1, WAVE, 0, o.st>>>(v, ub, eb, td, ix->d_values, rx, s, e, out, o.cap); break;
-        case ONE_UB: k_one_query<S, ONE_UB><<<1, WAVE, 0, o.st>>>(v, ub, eb, td, ix->d_values, rx, s, e, out, o.cap); break;
-        default: k_one_query<S, ONE_ANY><<<1, WAVE, 0, o.st>>>(v, ub, eb, td, ix->d_values, rx, s, e, out, o.cap); break;
+        case W_COUNT: k_one_query<S, W_COUNT><<<1, WAVE, 0, o.st>>>(v, ub, eb, td, lv, ix->d_values, rx, s, e, out, o.cap); break;
+        case W_DESC: k_one_query<S, W_DESC><<<1, WAVE, 0, o.st>>>(v, ub, eb, td, lv, ix->d_values, rx, s, e, out, o.cap); break;
+        case W_VECTOR: k_one_query<S, W_VECTOR><<<1, WAVE, 0, o.st>>>(v, ub, eb, td, lv, ix->d_values, rx, s, e, out, o.cap); break;
+        case W_VALUES: k_one_query<S, W_VALUES><<<1, WAVE, 0, o.st>>>(v, ub, eb, td, lv, ix->d_values, rx, s, e, out, o.cap); break;
+        case ONE_UB: k_one_query<S, ONE_UB><<<1, WAVE, 0, o.st>>>(v, ub, eb, td, lv, ix->d_values, rx, s, e, out, o.cap); break;
+        default: k_one_query<S, ONE_ANY><<<1, WAVE, 0, o.st>>>(v, ub, eb, td, lv, ix->d_values, rx, s, e, out, o.cap); break;
     }
     LAUNCH_CHECK();
     HIP_TRY(hipStreamSynchronize(o.st));

@@ -433,6 +433,56 @@ def test_wide_queries_guaranteed_run(sia, orc, dtype):
     ix.close()
 
 
+@pytest.mark.parametrize("dtype,long_len", [(np.int64, 0), (np.int64, 4096), (np.int32, 64), (np.int32, 1)])
+def test_long_list_walk(sia, orc, dtype, long_len):
+    """The wave walk with a long list: hits below the first start >= s - len come from the
+    list of intervals longer than len.  Built with the threshold chosen from the lengths
+    (0) or forced, queries on the wave walk (lane_steps 2) and not, the list used or not,
+    every mode and order against the oracle."""
+    from superintervals_amd import autotune
+    print(f"[gpu] long list dtype={np.dtype(dtype).name} long_len={long_len}", flush=True)
+    if dtype == np.int64:
+        s, e = sia.workload_intervals(3, 2_000_000, seed=33, sorted=False)
+        qs, qe = sia.workload_queries(3, 6000, seed=34)
+    else:
+        rng = np.random.default_rng(12)
+        n = 300_000
+        s = rng.integers(0, 20_000_000, n)
+        e = s + np.where(rng.random(n) < 0.1, rng.integers(5000, 2_000_000, n), rng.integers(-3, 300, n))
+        qs = rng.integers(0, 20_000_000, 6000)
+        qe = qs + rng.integers(-2, 3000, qs.size)
+        s, e, qs, qe = (x.astype(np.int32) for x in (s, e, qs, qe))
+    saved = autotune.current()
+    try:
+        autotune.apply({"long_len": long_len})
+        ix = _dev_index(sia, s, e, np.arange(s.size, dtype=np.int32))
+    finally:
+        autotune.apply(saved)
+    oix = orc.build_index(s, e)
+    wc = orc.count_batch(oix, qs, qe)
+    want = {o: orc.search_batch(oix, qs, qe, mode=m) for o, m in (("desc", 0), ("vector", 1))}
+    try:
+        for lw in (1, 0):
+            for ls in (2, 1024):
+                autotune.apply({"long_walk": lw, "lane_steps": ls, "count_rank": 0, "offsets_rank": 0})
+                assert np.array_equal(ix.count(qs, qe), wc), (lw, ls)
+                for order in ("desc", "vector"):
+                    off, idx = ix.search(qs, qe, order=order)
+                    assert np.array_equal(off, want[order][0]), (lw, ls, order)
+                    assert np.array_equal(idx.astype(np.int64), want[order][1]), (lw, ls, order)
+                off, vals = ix.search(qs, qe, values=True)
+                assert np.array_equal(vals, oix["data"][want["desc"][1]]), (lw, ls)
+        # single queries take the wave walk with the list too
+        autotune.apply({"long_walk": 1})
+        for q in range(0, 200, 7):
+            assert ix.count(qs[q:q + 1], qe[q:q + 1])[0] == wc[q]
+            off, idx = ix.search(qs[q:q + 1], qe[q:q + 1], order="vector")
+            assert np.array_equal(idx.astype(np.int64), orc.search(oix, qs[q], qe[q], 1))
+    finally:
+        autotune.apply(saved)
+    ix.close()
+
+
 def test_learned_index(sia, orc):
     """Learned upper_bound (workload 4 distributions, 2M intervals): GPU gradients against the
     float64 oracle, output identical to the plain search, and the trained root's gain over the
