@@ -228,6 +228,17 @@ def cpu_model():
     return "unknown"
 
 
+def calibrate_copy(dev):
+    """One 1 GiB device-to-device copy: the reference kernel for the PMC byte counters
+    (its FETCH_SIZE / WRITE_SIZE readings convert counter KiB to bytes)."""
+    import torch
+    src = torch.ones(1 << 28, dtype=torch.int32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def cpu_baseline(s, e, qs, qe, sample, check=None):
     """Reference CPU path on a bounded sample of the same queries, all host cores: search_idxs
     and count.  With check = (offsets, positions, query numbers) of the GPU's descending
@@ -397,6 +408,8 @@ def bench_skewed(a):
     from superintervals_amd import autotune
     dev = torch.device("cuda:0")
     torch.cuda.init()
+    if a.calibrate_copy:
+        calibrate_copy(dev)
     n, nq, bq = a.skewed_n, a.skewed_nq, a.skewed_batch
     t0 = time.perf_counter()
     s, e = sia.workload_intervals(3, n, seed=31, sorted=True)
@@ -616,11 +629,7 @@ def walk_once(a):
     dev = torch.device("cuda:0")
     torch.cuda.init()
     if a.calibrate_copy:
-        src = torch.ones(1 << 28, dtype=torch.int32, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        calibrate_copy(dev)
     knobs = tuning_arg(a)
     if knobs:
         autotune.apply(knobs)
@@ -755,11 +764,7 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     if a.calibrate_copy:
-        src = torch.ones(1 << 28, dtype=torch.int32, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        calibrate_copy(dev)
     s, e = sia.workload_intervals(2, a.n, seed=1, sorted=True)
     qs, qe = sia.workload_queries(2, a.nq, seed=2)
     ix = sia.DeviceIndex(np.int32).build(s, e)

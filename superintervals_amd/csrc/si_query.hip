@@ -55,9 +55,10 @@ struct HeavyEntry;
 
 template <typename S>
 struct PlanView {
-    uint32_t* counter;   // [0] tiles taken, [1] heavy entries written, [2] heavy entries taken,
-                         // [3] set by a fill pass when a query's hits differ from its offsets,
-                         // [4] 1 when the plan holds lower_bound positions (lb)
+    uint32_t* counter;   // [0] tiles taken, [1] heavy entries written, [3] set by a fill pass
+                         // when a query's hits differ from its offsets, [4] 1 when the plan
+                         // holds lower_bound positions (lb), [8 + c] entries taken from chunk c
+                         // of the heavy list (HEAVY_CHUNKS chunks)
     uint32_t* q;         // input query number of sorted position k
     S* s;                // query start of sorted position k
     S* e;                // query end of sorted position k
@@ -459,14 +460,45 @@ struct Seg {
 constexpr int EMIT_EVERY = SI_EMIT_EVERY;
 static_assert(EMIT_EVERY >= 2 && EMIT_EVERY <= 4, "2 to 4 segments per emission");
 
+// Range segments longer than this are written on their own (see emit_segments).
+constexpr uint32_t BIG_SEG = 1u << 23;
+static_assert((uint64_t)4 * 64 * BIG_SEG <= (1ull << 31), "the slots left for the scan fit in 32 bits");
+
 // Every lane holds up to N segments; they occupy consecutive slots of the concatenation.
+// Range segments longer than BIG_SEG (a run of millions of hits) are first written one at a
+// time by the whole wave, so that the slots of the rest, at most 64 x N x BIG_SEG, stay
+// within the 32-bit scan.
 template <typename S, int MODE, int N>
-__device__ __forceinline__ void emit_segments(const Seg (&sg)[N], int lane, EmitSlot* __restrict__ sd,
+__device__ __forceinline__ void emit_segments(const Seg (&sg_in)[N], int lane, EmitSlot* __restrict__ sd,
                                               const uint32_t* __restrict__ kbit,
                                               uint32_t* __restrict__ out_idx,
                                               const int32_t* __restrict__ vals,
                                               int32_t* __restrict__ out_val) {
     constexpr int W = Line<S>::W;
+    Seg sg[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) sg[j] = sg_in[j];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        uint64_t big = __ballot(sg[j].c > BIG_SEG);
+        while (big) {
+            const int L = __builtin_ctzll(big);
+            big &= big - 1ull;
+            // (readlane returns int: every word is taken as uint32_t before it is widened)
+            const uint32_t bc = (uint32_t)__builtin_amdgcn_readlane(sg[j].c, L);
+            const uint32_t bmk = (uint32_t)__builtin_amdgcn_readlane(sg[j].mk, L);
+            const uint32_t bbase = (uint32_t)__builtin_amdgcn_readlane(sg[j].base, L);
+            const uint64_t bo = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((uint32_t)(sg[j].o >> 32), L) << 32) |
+                                (uint64_t)(uint32_t)__builtin_amdgcn_readlane((uint32_t)sg[j].o, L);
+            const bool asc = (bmk >> 16) & 1u;
+            for (uint32_t x = (uint32_t)lane; x < bc; x += WAVE) {
+                const uint32_t p = asc ? bbase + x : bbase - x;
+                if (MODE == W_VALUES) out_val[bo + x] = vals[p];
+                else out_idx[bo + x] = p;
+            }
+            if (lane == L) sg[j].c = 0u;
+        }
+    }
     uint32_t c = 0;
 #pragma unroll
     for (int j = 0; j < N; ++j) c += sg[j].c;
@@ -525,6 +557,28 @@ __device__ __forceinline__ void emit_segments(const Seg (&sg)[N], int lane, Emit
             const uint64_t oo = ((uint64_t)d.o_hi << 32) | d.o_lo;
             if (MODE == W_VALUES) out_val[oo + r] = vals[pos];
             else out_idx[oo + r] = pos;
+        }
+        // The segment holding the chunk's last slot runs on into the next chunk.  When it is a
+        // range that covers whole chunks beyond this one, no other segment starts in them:
+        // they are written here with plain stores, and the loop goes on at the first chunk
+        // the range does not cover (which it still enters, from slot WAVE).
+        // (readlane returns int: every word is taken as uint32_t before it is widened)
+        const uint32_t lkv = (uint32_t)__builtin_amdgcn_readlane(d.kv, WAVE - 1);
+        if ((lkv >> 16) & 3u) {
+            const uint32_t lpre = (uint32_t)__builtin_amdgcn_readlane(d.pre, WAVE - 1);
+            const uint32_t lc = (uint32_t)__builtin_amdgcn_readlane(d.c, WAVE - 1);
+            const uint32_t lbase = (uint32_t)__builtin_amdgcn_readlane(d.base, WAVE - 1);
+            const uint64_t lo = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(d.o_hi, WAVE - 1) << 32) |
+                                (uint64_t)(uint32_t)__builtin_amdgcn_readlane(d.o_lo, WAVE - 1);
+            const uint32_t lend = min(lpre + lc, total);   // the range's end in the concatenation
+            const bool asc = (lkv >> 16) & 1u;
+            while (x0 + 2u * WAVE <= lend) {
+                x0 += WAVE;
+                const uint32_t rr = x0 + (uint32_t)lane - lpre;
+                const uint32_t p = asc ? lbase + rr : lbase - rr;
+                if (MODE == W_VALUES) out_val[lo + rr] = vals[p];
+                else out_idx[lo + rr] = p;
+            }
         }
     }
 }
@@ -855,8 +909,10 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
 // the current group of 64 lines come from one ballot over their level-1 maxima (kept as a
 // wave-uniform mask), and when the group has none left, the nearest group to the left with a
 // line >= s is found through the levels above (prev_line, the search build uses for branch).
-// Up to WAVE_LINES such lines are fetched per round trip.  The walk reads no branch or exit
-// array: ends (or the 32-bit copy of int64 ends) and the small level-1 array only.
+// Up to WAVE_LINES such lines (LONG_LINES in the long list) are fetched per round trip.
+// The walk reads no branch or exit array: ends (or the 32-bit copy of int64 ends) and the
+// small level-1 array only, plus, in the long list, the position of each entry, fetched in
+// the same round trip as its end (a whole line costs the same cache lines as its hits).
 //
 // In the C++ vector order the leading run [i* + 1, top] is written ascending once its bottom
 // i* (the first miss below top) is found: the lead phase steps through consecutive lines
@@ -869,7 +925,11 @@ __global__ __launch_bounds__(T_THREADS) SI_WALK_OCCUPANCY void k_walk(IndexView<
 #ifndef SI_WAVE_LINES
 #define SI_WAVE_LINES 4
 #endif
+#ifndef SI_LONG_LINES
+#define SI_LONG_LINES 8
+#endif
 constexpr uint32_t WAVE_LINES = SI_WAVE_LINES;   // lines with hits fetched per round trip
+constexpr uint32_t LONG_LINES = SI_LONG_LINES;   // the same in the long list (dense hits)
 
 // Last position whose key is <= x through a search tree (UbTree), by one wave, or NONE:
 // the lanes sample every 64th key of the top level (at most 4095 keys), then read the 64
@@ -946,74 +1006,157 @@ __device__ __forceinline__ uint64_t group_mask(const TreeDesc<S>& td, uint32_t g
     return __ballot(i < td.size[1] && td.lvl[1][i] >= s);
 }
 
+// Cache policy bits of the wave walk's position stores: 2 = non-temporal (the lists are
+// written once and not read back by the walk, so they need not hold L2 lines the walk's
+// reads of ends and positions could use); 0 = default.
+#ifndef SI_OUT_AUX
+#define SI_OUT_AUX 2
+#endif
+
+// Bits 0 .. x of a 64-bit mask (x <= 63).
+__device__ __forceinline__ uint64_t mask_upto(uint32_t x) { return (x >= 63u) ? ~0ull : ((2ull << x) - 1ull); }
+
 // The hits among entries [lo, cur] of an ends array with max tree td (td.lvl[0] = the ends),
-// highest first: one 64-entry line per step, the lines with a maximum >= s only.  The output
-// value of entry i is posmap[i] (the long list), or i itself.  Count and fill modes as
-// wave_walk.
-template <typename S, int MODE>
+// highest first: one 64-entry line per step, the lines with a maximum >= s only, NL lines
+// per round trip.  The line of cur and the line of lo are taken on their own, clipped to
+// [lo, cur]; the lines between need no clipping.  The output value of entry i is posmap[i]
+// (LONG: the long list), or i itself.  E32: the ends are read through their 32-bit copy
+// (end - base32), compared with s - base32 (0 when s <= base32; no hit at all when s is
+// more than 2^32 - 1 above base32), which selects the same entries.  acc, oend, left and the modes as wave_walk; the output offset of a hit within
+// this walk is a 32-bit count (a walk finds at most one hit per position).
+template <typename S, int MODE, bool LONG, bool E32, uint32_t NL>
 __device__ __forceinline__ void guided_walk(const S* __restrict__ ends, const uint32_t* __restrict__ ends32,
                                             int64_t base32, const uint32_t* __restrict__ posmap,
                                             const TreeDesc<S>& td, int lane, S s, uint32_t cur, uint32_t lo,
                                             uint64_t& acc, uint64_t oend, uint32_t& left,
                                             uint32_t* __restrict__ out_idx, const int32_t* __restrict__ vals,
                                             int32_t* __restrict__ out_val) {
-    const uint64_t above = (lane == 63) ? 0ull : (~0ull << (lane + 1));   // lanes above this one
-    auto take = [&](uint32_t b0, uint32_t kc, uint32_t klo, S v) {
-        const bool hit = ((uint32_t)lane <= kc) && ((uint32_t)lane >= klo) && (v >= s);
-        const uint64_t m = __ballot(hit);
-        const uint32_t nh = (uint32_t)__popcll(m);
-        if (MODE == W_COUNT) {
-            acc += nh;
-        } else {
-            left -= nh;
-            const uint64_t r = (uint64_t)__popcll(m & above);   // hits above this lane
-            if (hit && acc + r < oend) {
-                const uint32_t p = posmap ? posmap[b0 + lane] : b0 + (uint32_t)lane;
-                if (MODE == W_VALUES) out_val[acc + r] = vals[p];
-                else out_idx[acc + r] = p;
-            }
-            acc = min(acc + nh, oend);
-        }
-    };
     if (cur == NONE || cur < lo) return;
-    if (td.nlev < 2) {   // one line only (no level above the ends)
-        take(0u, cur, lo, end_at(ends, ends32, base32, (uint32_t)lane));
-        return;
+    uint32_t s32 = 0;
+    if (E32 && (int64_t)s > base32) {
+        const uint64_t d = (uint64_t)s - (uint64_t)base32;
+        if (d > 0xFFFFFFFFull) return;   // s is above every end of the array: no hit
+        s32 = (uint32_t)d;
     }
+    // fill modes: this walk writes out[acc + k] for k < room
+    uint32_t* const outq = (MODE == W_DESC || MODE == W_VECTOR) ? out_idx + acc : nullptr;
+    int32_t* const outv = (MODE == W_VALUES) ? out_val + acc : nullptr;
+    const uint32_t room = (MODE == W_COUNT) ? 0u : (uint32_t)min(oend - acc, (uint64_t)0xFFFFFFFFu);
+    uint32_t k = 0;   // hits found by this walk
+    // this lane's entry of line b0 (the long list's arrays are padded to whole lines)
+    auto load = [&](uint32_t b0, uint32_t& e32, S& ev, uint32_t& pv) {
+        if (E32) e32 = ends32[b0 + (uint32_t)lane];
+        else ev = ends[b0 + (uint32_t)lane];
+        if (MODE != W_COUNT) pv = LONG ? posmap[b0 + (uint32_t)lane] : b0 + (uint32_t)lane;
+    };
+    // Position lists are written through a buffer resource over out[acc + k0 ..] (k0: the
+    // hits found before the current round of lines) that holds the room left, at most 2^28
+    // values: every lane stores, and a lane without a hit (or past the room) stores at an
+    // offset out of the resource's range (0x80000000 at least), which the hardware drops, so
+    // the lines' takes need no branch.  The payload mode writes with a lane test instead.
+    auto out_rsrc = [&](uint32_t k0) {
+        const uint32_t nv = (k0 < room) ? min(room - k0, 1u << 28) : 0u;
+        return __builtin_amdgcn_make_buffer_rsrc((MODE == W_DESC || MODE == W_VECTOR) ? (void*)(outq + k0) : nullptr,
+                                                 (short)0, (int)(nv * 4u), 0x00020000);
+    };
+    // h: this lane's entry is a hit; the wave's hits are written highest lane first
+    auto take = [&](bool h, uint32_t pv, __amdgpu_buffer_rsrc_t rs, uint32_t k0) {
+        const uint64_t m = __builtin_amdgcn_ballot_w64(h);
+        const uint32_t cnt = (uint32_t)__popcll(m);
+        if (MODE != W_COUNT) {
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            const uint32_t r = k + cnt - 1u - below;
+            if (MODE == W_VALUES) {
+                if (h && r < room) outv[r] = vals[pv];
+            } else {
+                // (r - k0) * 4 < 2^31 for a hit; bit 31 set for the others
+                __builtin_amdgcn_raw_buffer_store_b32(pv, rs, ((r - k0) << 2) | (h ? 0u : 0x80000000u), 0, SI_OUT_AUX);
+            }
+        }
+        k += cnt;
+    };
+    auto is_hit = [&](uint32_t e32, S ev) -> bool { return E32 ? (e32 >= s32) : (ev >= s); };
+    // one line on its own, clipped to entries [klo, kc] of it
+    auto take_clipped = [&](uint32_t line, uint32_t kc, uint32_t klo) {
+        uint32_t e32 = 0, pv = 0;
+        S ev = 0;
+        load(line * 64u, e32, ev, pv);
+        take(((uint32_t)lane <= kc) && ((uint32_t)lane >= klo) && is_hit(e32, ev), pv, out_rsrc(k), k);
+    };
     const uint32_t L0 = cur >> 6;        // the first line, walked at and below cur only
     const uint32_t lo_line = lo >> 6;    // the last line, walked at and above lo only
-    uint32_t g1 = L0 & ~63u;             // group of 64 lines in hand
-    const uint32_t sh = L0 - g1;
-    uint64_t rem = group_mask(td, g1, s, lane) & ((sh == 63u) ? ~0ull : ((2ull << sh) - 1ull));
-    while (true) {
-        if (lo_line >= g1) rem &= ~((1ull << (lo_line - g1)) - 1ull);   // lines below lo_line are out
-        if (rem == 0) {
-            if (g1 <= lo_line) return;
-            const uint32_t l = prev_line(td, g1, lo_line, s, lane);
-            if (l == NONE) return;
-            g1 = l & ~63u;
-            rem = group_mask(td, g1, s, lane) & ((2ull << (l - g1)) - 1ull);   // lines of the group up to l
-            continue;
-        }
-        // the next WAVE_LINES lines with hits, highest first (NONE past the group's last)
-        uint32_t ln[WAVE_LINES];
-        S v[WAVE_LINES];
+    take_clipped(L0, cur - L0 * 64u, (L0 == lo_line) ? lo - L0 * 64u : 0u);
+    if (L0 > lo_line + 1u && td.nlev >= 2) {
+        // the lines strictly between lo_line and L0 whose maximum is >= s, highest first
+        const uint32_t hi_line = L0 - 1u;
+        uint32_t g1 = hi_line & ~63u;    // group of 64 lines in hand
+        uint64_t rem = group_mask(td, g1, s, lane) & mask_upto(hi_line - g1);
+        while (true) {
+            if (lo_line >= g1) rem &= ~mask_upto(lo_line - g1);   // lines up to lo_line are out
+            if (rem == 0) {
+                if (g1 <= lo_line + 1u) break;
+                const uint32_t l = prev_line(td, g1, lo_line + 1u, s, lane);
+                if (l == NONE) break;
+                g1 = l & ~63u;
+                rem = group_mask(td, g1, s, lane) & mask_upto(l - g1);   // lines of the group up to l
+                continue;
+            }
+            // The next NL lines with hits, highest first: line g1 + b[j].  The group's arrays
+            // are read through buffer resources based at its first entry (64 lines of 64), at
+            // byte b * 64 * width + lane * width.  A round with fewer lines left repeats its
+            // last line in the free slots, whose takes find no hit (valid false).
+            const uint32_t nv = min((uint32_t)__popcll(rem), NL);
+            uint32_t b[NL];
 #pragma unroll
-        for (uint32_t j = 0; j < WAVE_LINES; ++j) {
-            ln[j] = rem ? g1 + 63u - (uint32_t)__builtin_clzll(rem) : NONE;
-            rem = rem ? (rem & ~(1ull << (ln[j] - g1))) : 0ull;
-        }
+            for (uint32_t j = 0; j < NL; ++j) {
+                if (j < nv) {
+                    b[j] = 63u - (uint32_t)__builtin_clzll(rem);
+                    rem &= ~(1ull << b[j]);
+                } else {
+                    b[j] = b[j - 1u];
+                }
+            }
+            constexpr uint32_t EW = E32 ? 4u : (uint32_t)sizeof(S);   // bytes per end read
+            const __amdgpu_buffer_rsrc_t re = __builtin_amdgcn_make_buffer_rsrc(
+                E32 ? (void*)(ends32 + (size_t)g1 * 64u) : (void*)(ends + (size_t)g1 * 64u), (short)0, (int)(4096u * EW),
+                0x00020000);
+            const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(
+                (LONG && MODE != W_COUNT) ? (void*)(posmap + (size_t)g1 * 64u) : nullptr, (short)0, (int)(4096u * 4u),
+                0x00020000);
+            uint32_t e32[NL], pv[NL];
+            S ev[NL];
 #pragma unroll
-        for (uint32_t j = 0; j < WAVE_LINES; ++j) {
-            const uint32_t b0 = (ln[j] == NONE) ? 0u : ln[j] * 64u;   // absent lines read line 0, unused
-            v[j] = end_at(ends, ends32, base32, b0 + (uint32_t)lane);
-        }
+            for (uint32_t j = 0; j < NL; ++j) {
+                if (E32) {
+                    e32[j] = __builtin_amdgcn_raw_buffer_load_b32(re, (uint32_t)lane * 4u, b[j] * 256u, 0);
+                } else if (sizeof(S) == 8) {
+                    const auto w = __builtin_amdgcn_raw_buffer_load_b64(re, (uint32_t)lane * 8u, b[j] * 512u, 0);
+                    ev[j] = (S)(((uint64_t)(uint32_t)w[1] << 32) | (uint64_t)(uint32_t)w[0]);
+                } else {
+                    ev[j] = (S)__builtin_amdgcn_raw_buffer_load_b32(re, (uint32_t)lane * 4u, b[j] * 256u, 0);
+                }
+                if (MODE != W_COUNT)
+                    pv[j] = LONG ? __builtin_amdgcn_raw_buffer_load_b32(rp, (uint32_t)lane * 4u, b[j] * 256u, 0)
+                                 : (g1 + b[j]) * 64u + (uint32_t)lane;
+            }
+            // every line is taken, so no branch holds back a load
+            const uint32_t k0 = k;
+            const __amdgpu_buffer_rsrc_t rs = out_rsrc(k0);
+            if (nv == NL) {
 #pragma unroll
-        for (uint32_t j = 0; j < WAVE_LINES; ++j) {
-            if (ln[j] == NONE) continue;
-            const uint32_t b0 = ln[j] * 64u;
-            take(b0, (ln[j] == L0) ? cur - b0 : 63u, (ln[j] == lo_line) ? lo - b0 : 0u, v[j]);
+                for (uint32_t j = 0; j < NL; ++j) take(is_hit(e32[j], ev[j]), pv[j], rs, k0);
+            } else {
+#pragma unroll
+                for (uint32_t j = 0; j < NL; ++j) take(j < nv && is_hit(e32[j], ev[j]), pv[j], rs, k0);
+            }
         }
+    }
+    if (L0 > lo_line) take_clipped(lo_line, 63u, lo - lo_line * 64u);
+    if (MODE == W_COUNT) {
+        acc += k;
+    } else {
+        left -= k;
+        acc += min(k, room);
     }
 }
 
@@ -1058,18 +1201,41 @@ __device__ __forceinline__ void wave_walk(const IndexView<S>& ix, const TreeDesc
         // x = s - len - 1 (upper_bound(x) + 1 = the first position starting at >= s - len)
         if ((int64_t)s > (int64_t)SMIN + lv.len) P0 = wave_tree_ub(ub, (S)((int64_t)s - lv.len - 1), lane) + 1u;
     }
-    guided_walk<S, MODE>(ix.ends, ix.ends32, ix.ends32_base, nullptr, td, lane, s, cur, P0, acc, oend, left, out_idx,
-                         vals, out_val);
+    if (sizeof(S) == 8 && ix.ends32)
+        guided_walk<S, MODE, false, true, WAVE_LINES>(ix.ends, ix.ends32, ix.ends32_base, nullptr, td, lane, s, cur, P0,
+                                                      acc, oend, left, out_idx, vals, out_val);
+    else
+        guided_walk<S, MODE, false, false, WAVE_LINES>(ix.ends, nullptr, 0, nullptr, td, lane, s, cur, P0, acc, oend,
+                                                       left, out_idx, vals, out_val);
     if (P0 > 0) {
         const uint32_t J = lv.lrank[min(P0, cur + 1u)];   // long entries below that position
-        if (J > 0)
-            guided_walk<S, MODE>(lv.ends, lv.ends32, lv.ends32_base, lv.pos, lv.td, lane, s, J - 1u, 0u, acc, oend,
-                                 left, out_idx, vals, out_val);
+        if (J > 0 && sizeof(S) == 8 && lv.ends32)
+            guided_walk<S, MODE, true, true, LONG_LINES>(lv.ends, lv.ends32, lv.ends32_base, lv.pos, lv.td, lane, s,
+                                                         J - 1u, 0u, acc, oend, left, out_idx, vals, out_val);
+        else if (J > 0)
+            guided_walk<S, MODE, true, false, LONG_LINES>(lv.ends, nullptr, 0, lv.pos, lv.td, lane, s, J - 1u, 0u, acc,
+                                                          oend, left, out_idx, vals, out_val);
     }
 }
 
 // Continuation of walks the lane walk parked (over its step budget): one wave per parked
-// query, taken from a global counter.
+// query.  The heavy list holds the parked queries roughly in end order; it is cut into
+// HEAVY_CHUNKS contiguous chunks, each with its own counter, and workgroup w starts on chunk
+// w % HEAVY_CHUNKS, then takes what is left of the others.  Workgroups are dispatched round
+// robin over the chip's 8 XCDs, so with 8 chunks each XCD's L2 serves the queries of one
+// stretch of the coordinates (the results never depend on which wave takes an entry).
+#ifndef SI_HEAVY_CHUNKS
+#define SI_HEAVY_CHUNKS 8
+#endif
+constexpr uint32_t HEAVY_CHUNKS = SI_HEAVY_CHUNKS;
+static_assert(HEAVY_CHUNKS >= 1 && HEAVY_CHUNKS <= 8, "the plan holds 8 chunk counters");
+
+// Clears the counters a walk launch uses (tiles, heavy list, mismatch flag, chunk counters).
+__global__ void k_walk_counters_reset(uint32_t* __restrict__ c) {
+    const uint32_t i = threadIdx.x;
+    if (i < 4u || (i >= 8u && i < 8u + HEAVY_CHUNKS)) c[i] = 0u;
+}
+
 template <typename S, int MODE, bool MULTI = false>
 __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix_arg, TreeDesc<S> td_arg, UbTree<S> ub_arg,
                                                           LongView<S> lv, PlanView<S> pv,
@@ -1086,13 +1252,20 @@ __global__ __launch_bounds__(Q_THREADS) void k_walk_heavy(IndexView<S> ix_arg, T
     const int32_t* __restrict__ vals = vals_arg;
     const int lane = lane_id();
     const uint32_t nheavy = __builtin_amdgcn_readfirstlane(pv.counter[1]);
-    if (nheavy == 0) return;   // nothing parked: no wave touches the shared counter
+    if (nheavy == 0) return;   // nothing parked: no wave touches the shared counters
+    const uint32_t chunk = (nheavy + HEAVY_CHUNKS - 1u) / HEAVY_CHUNKS;
+    uint32_t c = blockIdx.x % HEAVY_CHUNKS, tried = 0;
     while (true) {
         uint32_t h = 0;
-        if (lane == 0) h = atomicAdd(&pv.counter[2], 1u);
+        if (lane == 0) h = atomicAdd(&pv.counter[8 + c], 1u);
         h = __builtin_amdgcn_readfirstlane(__shfl(h, 0, WAVE));
-        if (h >= nheavy) break;
-        const HeavyEntry he = pv.heavy[h];
+        const uint32_t beg = c * chunk, len = (beg < nheavy) ? min(chunk, nheavy - beg) : 0u;
+        if (h >= len) {   // chunk c is exhausted: the next one
+            if (++tried == HEAVY_CHUNKS) break;
+            c = (c + 1u) % HEAVY_CHUNKS;
+            continue;
+        }
+        const HeavyEntry he = pv.heavy[beg + h];
         const uint32_t k = __builtin_amdgcn_readfirstlane(he.k);
         if (MULTI) {   // the parked query's set
             const uint32_t g = __builtin_amdgcn_readfirstlane(pv.set[k]);
@@ -1499,7 +1672,7 @@ int launch_walk(const siIndex* ix, void* plan, size_t nq, uint32_t* counts, uint
     uint32_t tile = 0, ntiles = 0;
     unsigned grid = 0;
     walk_shape<S, MODE>(nq, &tile, &ntiles, &grid);
-    k_zero_u32<<<1, 64, 0, st>>>(pv.counter, 4);
+    k_walk_counters_reset<<<1, 64, 0, st>>>(pv.counter);
     LAUNCH_CHECK();
     k_walk<S, MODE><<<grid, T_THREADS, 0, st>>>(
         ix->view<S>(), pv, (uint32_t)nq, tile, ntiles, g_tune.lane_steps, range_skip_on(ix), pv.counter, counts,
@@ -2197,7 +2370,7 @@ int launch_walk_multi(const siSetTable* tb, void* plan, size_t nq, const uint64_
     const uint32_t tile = multi_tile();
     const size_t tiles_max = (nq + tile - 1) / tile + tb->nslots;
     const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>((size_t)cus * wgs, tiles_max));
-    k_zero_u32<<<1, 64, 0, st>>>(pv.counter, 4);
+    k_walk_counters_reset<<<1, 64, 0, st>>>(pv.counter);
     LAUNCH_CHECK();
     const SetView<int32_t>* sets = (const SetView<int32_t>*)tb->d_views;
     k_walk<int32_t, MODE, true><<<grid, T_THREADS, 0, st>>>(

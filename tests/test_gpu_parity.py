@@ -1024,3 +1024,40 @@ def test_full_size_lognormal_10Mx10M_properties(sia, orc):
         assert np.array_equal(idxh[offh[q]:offh[q + 1]].astype(np.int64), oidx[ooff[k]:ooff[k + 1]])
     del d
     ix.close()
+
+
+def test_output_offsets_beyond_2_31(sia):
+    """Lists whose output offsets pass 2^31 entries (8.7 GB of positions) on n = 2^24 + 5
+    intervals [i, n + i], with queries in triples: [0, n - 1] hits all n as one guaranteed
+    run longer than 2^23 (written on its own by the whole wave), [0, 2^21 - 1] hits 2^21 as
+    a run of whole 64-slot chunks, and the point 2^23, whose 2^23 hits below its run are
+    walked by the wave-cooperative kernel.  Both orders; every entry is checked against the
+    closed form."""
+    import torch
+    print("[gpu] output offsets beyond 2^31", flush=True)
+    n, r2, p3, trip = (1 << 24) + 5, 1 << 21, 1 << 23, 80
+    s = np.arange(n, dtype=np.int32)
+    e = (s + n).astype(np.int32)
+    ix = _dev_index(sia, s, e)
+    qs = np.zeros(3 * trip, np.int32)
+    qe = np.tile(np.array([n - 1, r2 - 1, p3], np.int32), trip)
+    qs[2::3] = p3
+    dev = torch.device("cuda:0")
+    tqs, tqe = torch.from_numpy(qs).to(dev), torch.from_numpy(qe).to(dev)
+    off = torch.empty(qs.size + 1, dtype=torch.int64, device=dev)
+    ix.offsets_device(tqs, tqe, off)
+    torch.cuda.synchronize()
+    per = n + r2 + p3 + 1
+    total = int(off[-1].item())
+    assert total == trip * per and total > (1 << 31)
+    idx = torch.empty(total, dtype=torch.int32, device=dev)
+    rows = idx.view(trip, per)
+    desc = torch.cat([torch.arange(n - 1, -1, -1), torch.arange(r2 - 1, -1, -1), torch.arange(p3, -1, -1)])
+    vec = torch.cat([torch.arange(n), torch.arange(r2), torch.arange(p3 + 1)])
+    for order, row in (("desc", desc), ("vector", vec)):
+        idx.fill_(-1)
+        ix.fill_device(tqs, tqe, off, idx, order=order)
+        torch.cuda.synchronize()
+        assert bool((rows == row.to(torch.int32).to(dev)).all()), order
+    del rows, idx
+    ix.close()
