@@ -521,6 +521,25 @@ def bench_skewed(a):
     assert tot == hits, (tot, hits)
     print("[skewed] streamed search: %d batches, %.3g positions, fill %.1f ms, wall %.2f s" % (nb, tot, t_fill, wall),
           file=sys.stderr, flush=True)
+    # roofline of the streamed fill: the bytes it must move (positions written, plan entry and
+    # offset read per query) per second; the counter-measured traffic of one batch's fill
+    # launches, when the summary of a PMC run is committed
+    alg = 4 * tot + (2 * 8 + 4 + 8 + 8) * nq
+    ach = alg / (t_fill / 1e3) / 1e9
+    roof = {"bound": "hbm", "kernel": "k_walk<long, desc fill> + k_walk_heavy<long, desc fill> (streamed fill)",
+            "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "basis": "achieved = (4 B per position written + 36 B of plan entry, offsets and query per query) / "
+                     "fill time; traffic = FETCH_SIZE + WRITE_SIZE of one 1M-query batch's fill launches"}
+    tp = os.path.join(ROOT, "profiles", "pmc_traffic_search_skewed_50Mx20M.json")
+    if os.path.exists(tp):
+        with open(tp) as f:
+            tr = json.load(f)
+        per_batch = tr["fill_lane"]["bytes_per_launch"] + tr["fill_heavy"]["bytes_per_launch"]
+        roof["traffic"] = per_batch
+        roof["measured_GBps"] = round(per_batch * nb / (t_fill / 1e3) / 1e9, 1)
+        roof["measured_frac"] = round(roof["measured_GBps"] / HBM_PEAK_GBS, 4)
+        roof["traffic_profile"] = "profiles/pmc_traffic_search_skewed_50Mx20M.json"
     out = {"metric": "skewed int64 batch: count queries/s (one launch) and streamed search_idxs positions/s",
            "value": nq / (t_walk / 1e3), "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
            "dtype": "int64", "data": "synthetic (seeded hotspot-mixture generator, inputs resident in HBM)",
@@ -534,7 +553,8 @@ def bench_skewed(a):
            "lane_steps_count_walk_ms": {str(k): v for k, v in csweep.items()},
            "fill_batch_ms_ends32": ends32_ms.get(1), "fill_batch_ms_int64_ends": ends32_ms.get(0),
            "search": {"batches": nb, "fill_ms_total": t_fill, "positions_per_s": tot / (t_fill / 1e3),
-                      "wall_s_including_plan_offsets_sync": wall}}
+                      "wall_s_including_plan_offsets_sync": wall},
+           "roofline": roof}
     print(json.dumps(out))
 
 
