@@ -57,19 +57,21 @@ template <typename S>
 struct PlanView {
     uint32_t* counter;   // [0] tiles taken, [1] heavy entries written, [3] set by a fill pass
                          // when a query's hits differ from its offsets, [4] 1 when the plan
-                         // holds lower_bound positions (lb), [8 + c] entries taken from chunk c
-                         // of the heavy list (HEAVY_CHUNKS chunks)
+                         // holds lower_bound positions (lb), [5] 1 when it holds the counts
+                         // (cnt), [8 + c] entries taken from chunk c of the heavy list
+                         // (HEAVY_CHUNKS chunks)
     uint32_t* q;         // input query number of sorted position k
     S* s;                // query start of sorted position k
     S* e;                // query end of sorted position k
     uint32_t* idx;       // upper_bound of sorted position k
     uint32_t* lb;        // first position whose start is >= the query start (lower_bound)
+    uint32_t* cnt;       // the count of sorted position k (rank form), when counter[5] is set
     uint32_t* set;       // plans over a set table: the set of sorted position k
     HeavyEntry* heavy;   // parked walks (see HeavyEntry)
 };
 
 // Byte layout of a plan of nq queries (coordinate width cb):
-//   [256 B counters][q: 4 nq][s: cb nq][e: cb nq][idx: 4 nq][lb: 4 nq][scratch: keys 2 x 8 nq,
+//   [256 B counters][q: 4 nq][s: cb nq][e: cb nq][idx: 4 nq][lb: 4 nq][cnt: 4 nq][scratch: keys 2 x 8 nq,
 //   vals 2 x 4 nq, radix temporaries]; the scratch is reused as the offsets scan workspace.
 // A query whose walk exceeds the per-lane step budget, parked for the wave-cooperative
 // kernel: sorted position, current position, leading-run top and phase, the count so far
@@ -86,15 +88,17 @@ struct HeavyEntry {
 };
 
 struct PlanLayout {
-    size_t off_q, off_s, off_e, off_idx, off_lb, off_scratch, off_k0, off_k1, off_v0, off_v1, off_rs, off_heavy, total;
-    constexpr PlanLayout(int cb, size_t nq) : off_q(0), off_s(0), off_e(0), off_idx(0), off_lb(0), off_scratch(0), off_k0(0),
-                                             off_k1(0), off_v0(0), off_v1(0), off_rs(0), off_heavy(0), total(0) {
+    size_t off_q, off_s, off_e, off_idx, off_lb, off_cnt, off_scratch, off_k0, off_k1, off_v0, off_v1, off_rs, off_heavy,
+        total;
+    constexpr PlanLayout(int cb, size_t nq) : off_q(0), off_s(0), off_e(0), off_idx(0), off_lb(0), off_cnt(0), off_scratch(0),
+                                             off_k0(0), off_k1(0), off_v0(0), off_v1(0), off_rs(0), off_heavy(0), total(0) {
         off_q = 256;
         off_s = off_q + align256(4 * nq);
         off_e = off_s + align256((size_t)cb * nq);
         off_idx = off_e + align256((size_t)cb * nq);
         off_lb = off_idx + align256(4 * nq);
-        off_scratch = off_lb + align256(4 * nq);
+        off_cnt = off_lb + align256(4 * nq);
+        off_scratch = off_cnt + align256(4 * nq);
         off_k0 = off_scratch;
         off_k1 = off_k0 + align256(8 * nq);
         off_v0 = off_k1 + align256(8 * nq);
@@ -151,6 +155,7 @@ PlanView<S> plan_view(void* base, size_t nq) {
     v.e = (S*)(p + L.off_e);
     v.idx = (uint32_t*)(p + L.off_idx);
     v.lb = (uint32_t*)(p + L.off_lb);
+    v.cnt = (uint32_t*)(p + L.off_cnt);
     v.set = nullptr;
     v.heavy = (HeavyEntry*)(p + L.off_heavy);
     return v;
@@ -186,11 +191,13 @@ template <typename S>
 __global__ __launch_bounds__(Q_THREADS) void k_plan_finish(IndexView<S> ix, const uint64_t* __restrict__ key,
                                                            const uint32_t* __restrict__ val,
                                                            const S* __restrict__ qs, size_t nq,
-                                                           PlanView<S> pv, bool with_ub, bool with_lb) {
+                                                           PlanView<S> pv, bool with_ub, bool with_lb,
+                                                           bool with_rank) {
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) {
         pv.counter[0] = 0;
         pv.counter[4] = with_lb ? 1u : 0u;
+        pv.counter[5] = with_rank ? 1u : 0u;
     }
     if (k >= nq) return;
     uint64_t kk = key[k];
@@ -219,9 +226,13 @@ __global__ __launch_bounds__(Q_THREADS) void k_plan_finish(IndexView<S> ix, cons
 template <typename S>
 int launch_ub_tree(const siIndex* ix, const S* xs, size_t n, uint32_t* out, hipStream_t st);
 
-// The plan's upper_bound (and lower_bound) positions through the same tree (below).
+// The plan's upper_bound (and lower_bound) positions, and its rank-form counts, through the
+// search trees (below).
 template <typename S>
-int launch_plan_bounds(const siIndex* ix, PlanView<S> pv, size_t nq, bool with_lb, hipStream_t st);
+int launch_plan_bounds(const siIndex* ix, PlanView<S> pv, size_t nq, bool with_lb, bool with_rank, hipStream_t st);
+
+// The rank form needs no interval with start > end (defined with the launch helpers below).
+inline bool rank_exact(const siIndex* ix);
 
 // Number of sorted ends below x (branchless lower_bound).
 template <typename S>
@@ -349,11 +360,15 @@ int plan_impl(const siIndex* ix, const S* qs, const S* qe, size_t nq, void* plan
     const bool learned = g_tune.ub_learned && ix->learned;
     const bool tree = !learned && g_tune.ub_tree && ix->ub.d_top && ix->n > 0;
     const bool with_lb = range_skip_on(ix);
+    // with both trees of the same shape the bound search also finds each query's rank-form
+    // count, which the offsets and count passes then take from the plan
+    const bool with_rank = tree && rank_exact(ix) && ix->eb.d_top && ix->eb.nlv == ix->ub.nlv &&
+                           ix->eb.top_h == ix->ub.top_h;
     const PlanView<S> pv = plan_view<S>(plan, nq);
-    k_plan_finish<S><<<g, Q_THREADS, 0, st>>>(ix->view<S>(), ks, vs, qs, nq, pv, !tree && !learned, with_lb);
+    k_plan_finish<S><<<g, Q_THREADS, 0, st>>>(ix->view<S>(), ks, vs, qs, nq, pv, !tree && !learned, with_lb, with_rank);
     LAUNCH_CHECK();
     if (learned) return si_learned_plan_bounds(ix, pv.e, pv.s, nq, pv.idx, with_lb ? pv.lb : nullptr, st);
-    if (tree) return launch_plan_bounds<S>(ix, pv, nq, with_lb, st);
+    if (tree) return launch_plan_bounds<S>(ix, pv, nq, with_lb, with_rank, st);
     return SI_OK;
 }
 
@@ -1440,7 +1455,9 @@ __global__ __launch_bounds__(Q_THREADS) void k_count_rank_tree(IndexView<S> ix, 
     extern __shared__ __align__(16) char s_mem[];
     S* s_top = reinterpret_cast<S*>(s_mem);
     const uint32_t T = (1u << h) - 1u;
-    for (uint32_t k = threadIdx.x; k < T; k += Q_THREADS) s_top[k] = top[k];
+    const bool planned = __builtin_amdgcn_readfirstlane(pv.counter[5]) != 0u;   // the plan holds the counts
+    if (!planned)
+        for (uint32_t k = threadIdx.x; k < T; k += Q_THREADS) s_top[k] = top[k];
     __syncthreads();
     constexpr S SMIN = (sizeof(S) == 4) ? (S)INT32_MIN : (S)INT64_MIN;
     for (size_t k0 = (size_t)blockIdx.x * Q_THREADS; k0 < nq; k0 += (size_t)gridDim.x * Q_THREADS) {
@@ -1449,7 +1466,9 @@ __global__ __launch_bounds__(Q_THREADS) void k_count_rank_tree(IndexView<S> ix, 
         const uint32_t i = pv.idx[k];
         const S s = pv.s[k], e = pv.e[k];
         uint32_t c = 0;
-        if (i != NONE) {
+        if (planned) {
+            c = pv.cnt[k];
+        } else if (i != NONE) {
             if (s <= e) {
                 const uint32_t below = (s == SMIN) ? 0u : ub_tree_pos(t, s_top, h, (S)(s - 1)) + 1u;
                 c = i + 1u - below;
@@ -1498,33 +1517,123 @@ int launch_ub_tree(const siIndex* ix, const S* xs, size_t n, uint32_t* out, hipS
     return SI_OK;
 }
 
-// The plan's bounds through the tree: idx[k] = upper_bound(e[k]) and, with_lb, lb[k] = the
-// first position whose start is >= s[k].  One thread per sorted query, two descents.
+// Number of keys <= x of a tree's top level (Eytzinger order in LDS, h levels), clamped
+// to the level's size (padding keys are max(S)).
 template <typename S>
-__global__ __launch_bounds__(Q_THREADS) void k_plan_bounds_tree(UbTree<S> t, const S* __restrict__ top, uint32_t h,
-                                                                PlanView<S> pv, size_t nq, bool with_lb) {
+__device__ __forceinline__ uint32_t top_rank(const S* s_top, uint32_t h, uint32_t size, S x) {
+    uint32_t k = 1, r = 0;
+    for (uint32_t lvl = 0; lvl < h; ++lvl) {
+        const bool right = s_top[k - 1] <= x;
+        r += right ? (1u << (h - 1u - lvl)) : 0u;
+        k = 2u * k + (right ? 1u : 0u);
+    }
+    return min(r, size);
+}
+
+// The plan's bounds and counts: idx[k] = upper_bound(e[k]); with LB, lb[k] = the first
+// position whose start is >= s[k] (upper_bound(s - 1) + 1, 0 when s is the smallest value);
+// with RANK, cnt[k] = the rank-form count (upper_bound + 1 - #{ends < s}, #{ends < s} =
+// upper_bound(ends_sorted, s - 1) + 1), or the reference walk's count for s > e.  One thread
+// per sorted query; its searches descend the starts tree (twice) and the sorted-ends tree
+// level by level together, so their reads overlap.  (Reading each node's 8-key boundaries
+// in one round and then one 8-key segment, in place of halving, was slower: neighbouring
+// sorted queries share nodes, so the halving reads mostly hit L1, and the wider rounds cost
+// registers and instructions.)
+constexpr int PB_THREADS = 512;
+
+template <typename S, bool LB, bool RANK>
+__global__ __launch_bounds__(PB_THREADS) void k_plan_bounds_tree(IndexView<S> ix, UbTree<S> ts,
+                                                                 const S* __restrict__ tops, UbTree<S> te,
+                                                                 const S* __restrict__ tope, uint32_t h,
+                                                                 PlanView<S> pv, size_t nq) {
+    constexpr int N = 1 + (LB ? 1 : 0) + (RANK ? 1 : 0);
+    constexpr S SMIN = (sizeof(S) == 4) ? (S)INT32_MIN : (S)INT64_MIN;
     extern __shared__ __align__(16) char s_mem[];
-    S* s_top = reinterpret_cast<S*>(s_mem);
     const uint32_t T = (1u << h) - 1u;
-    for (uint32_t k = threadIdx.x; k < T; k += Q_THREADS) s_top[k] = top[k];
+    S* s_ts = reinterpret_cast<S*>(s_mem);
+    S* s_te = s_ts + T;
+    for (uint32_t k = threadIdx.x; k < T; k += PB_THREADS) {
+        s_ts[k] = tops[k];
+        if (RANK) s_te[k] = tope[k];
+    }
     __syncthreads();
-    for (size_t k0 = (size_t)blockIdx.x * Q_THREADS; k0 < nq; k0 += (size_t)gridDim.x * Q_THREADS) {
+    for (size_t k0 = (size_t)blockIdx.x * PB_THREADS; k0 < nq; k0 += (size_t)gridDim.x * PB_THREADS) {
         const size_t k = k0 + threadIdx.x;
         if (k >= nq) continue;
-        pv.idx[k] = ub_tree_pos(t, s_top, h, pv.e[k]);
-        if (with_lb) {
-            const S s = pv.s[k];
-            pv.lb[k] = (s == smin_of<S>()) ? 0u : ub_tree_pos(t, s_top, h, (S)(s - 1)) + 1u;
+        const S e = pv.e[k], s = pv.s[k];
+        const bool s1 = s != SMIN;   // s - 1 exists
+        const S x1 = s1 ? (S)(s - 1) : s;
+        // searches: 0 = upper_bound(e), then lower bound (starts at s - 1), then #{ends < s}
+        const S* v[N];
+        UbTree<S> const* tr[N];
+        const S* tp[N];
+        S x[N];
+        int n = 0;
+        tr[n] = &ts; tp[n] = s_ts; x[n] = e; ++n;
+        if (LB) { tr[n] = &ts; tp[n] = s_ts; x[n] = x1; ++n; }
+        if (RANK) { tr[n] = &te; tp[n] = s_te; x[n] = x1; ++n; }
+        int64_t j[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) j[i] = (int64_t)top_rank(tp[i], h, tr[i]->size[tr[i]->nlv], x[i]) - 1;
+        for (int l = ts.nlv - 1; l >= 0; --l) {
+            uint32_t b[N], sz[N], out[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                v[i] = tr[i]->lv[l];
+                sz[i] = tr[i]->size[l];
+                b[i] = (uint32_t)(j[i] < 0 ? 0 : j[i]) * 64u;   // a search with no key <= x reads node 0, unused
+            }
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                out[i] = (uint32_t)last_le_in(v[i], (int64_t)b[i], (int64_t)min(b[i] + 63u, sz[i] - 1u), x[i]);
+#pragma unroll
+            for (int i = 0; i < N; ++i) j[i] = (j[i] < 0) ? j[i] : (int64_t)out[i];
+        }
+        const uint32_t ub = (j[0] < 0) ? NONE : (uint32_t)j[0];
+        pv.idx[k] = ub;
+        if (LB) pv.lb[k] = (!s1 || j[1] < 0) ? 0u : (uint32_t)j[1] + 1u;
+        if (RANK) {
+            const int64_t jr = j[N - 1];
+            uint32_t c = 0;
+            if (ub != NONE) {
+                if (s <= e) c = ub + 1u - ((!s1 || jr < 0) ? 0u : (uint32_t)jr + 1u);
+                else c = walk_count(ix, ub, s);
+            }
+            pv.cnt[k] = c;
         }
     }
 }
 
 template <typename S>
-int launch_plan_bounds(const siIndex* ix, PlanView<S> pv, size_t nq, bool with_lb, hipStream_t st) {
+int launch_plan_bounds(const siIndex* ix, PlanView<S> pv, size_t nq, bool with_lb, bool with_rank, hipStream_t st) {
     const KeyTree& kt = ix->ub;
-    const size_t lds = (size_t)((1u << kt.top_h) - 1u) * ix->cb;
-    k_plan_bounds_tree<S><<<tree_grid(nq), Q_THREADS, lds, st>>>(ub_tree_of<S>(kt, (const S*)ix->d_starts, ix->n),
-                                                                  (const S*)kt.d_top, kt.top_h, pv, nq, with_lb);
+    const KeyTree& ke = ix->eb;
+    const uint32_t T = (1u << kt.top_h) - 1u;
+    const size_t lds = (size_t)T * ix->cb * (with_rank ? 2u : 1u);
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        cus = std::max(cus, 1);
+    }
+    const UbTree<S> ts = ub_tree_of<S>(kt, (const S*)ix->d_starts, ix->n);
+    const UbTree<S> te = with_rank ? ub_tree_of<S>(ke, (const S*)ix->d_ends_sorted, ix->n) : ts;
+    const S* tope = with_rank ? (const S*)ke.d_top : nullptr;
+    // persistent: the resident blocks (registers and the staged tops allow) stage the tops once
+#define SI_PLAN_BOUNDS(LBV, RKV)                                                                                   \
+    do {                                                                                                           \
+        int per = 0;                                                                                               \
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_plan_bounds_tree<S, LBV, RKV>, PB_THREADS, lds);     \
+        const unsigned grid = (unsigned)std::min<size_t>(grid_for(nq, PB_THREADS), (size_t)cus * std::max(per, 1)); \
+        k_plan_bounds_tree<S, LBV, RKV><<<grid, PB_THREADS, lds, st>>>(ix->view<S>(), ts, (const S*)kt.d_top, te,   \
+                                                                       tope, kt.top_h, pv, nq);                     \
+    } while (0)
+    if (with_lb && with_rank) SI_PLAN_BOUNDS(true, true);
+    else if (with_lb) SI_PLAN_BOUNDS(true, false);
+    else if (with_rank) SI_PLAN_BOUNDS(false, true);
+    else SI_PLAN_BOUNDS(false, false);
+#undef SI_PLAN_BOUNDS
     LAUNCH_CHECK();
     return SI_OK;
 }
@@ -2279,6 +2388,7 @@ __global__ __launch_bounds__(Q_THREADS) void k_multi_finish(const SetView<int32_
     if (k == 0) {
         pv.counter[0] = 0;
         pv.counter[4] = 1u;   // lower bounds present (NONE where the set has start > end)
+        pv.counter[5] = 0u;   // counts are not kept in the plan
     }
     if (k >= nq) return;
     const uint64_t kk = key[k];
