@@ -53,6 +53,8 @@ def parse():
     p.add_argument("--skewed-n", type=int, default=50_000_000)
     p.add_argument("--skewed-nq", type=int, default=20_000_000)
     p.add_argument("--skewed-batch", type=int, default=1_000_000)
+    p.add_argument("--skewed-check", type=int, default=200,
+                   help="queries of the skewed run whose lists are checked against the oracle (0: none)")
     p.add_argument("--graph", action="store_true",
                    help="small batches: eager plan/offsets/fill vs one captured graph replay")
     p.add_argument("--graph-nq", type=int, default=4096)
@@ -288,6 +290,40 @@ def cpu_baseline(s, e, qs, qe, sample, check=None):
     return out
 
 
+def cpu_baseline_sampled(s, e, qs, qe, picks, gpu_lists):
+    """The reference's own search_idxs (its headers, oracle/_ref; else the oracle's restatement)
+    on the sampled queries picks of a workload, all host cores, timed (median of 3), and each
+    picked query's descending list compared with the GPU's (gpu_lists: query -> positions)."""
+    import numpy as np
+    import oracle
+    threads = len(os.sched_getaffinity(0))
+    sq, se = qs[picks], qe[picks]
+    t0 = time.perf_counter()
+    if oracle.Reference.available():
+        m = oracle.RefMap(oracle.Reference(), s, e)
+        kind, desc = "reference", "si::IntervalMap<int64_t,int>::search_idxs(s, e, std::vector&) from the reference headers"
+        run = lambda: m.search_batch_total(sq, se, threads)
+        lists = lambda k: np.asarray(m.search(sq[k], se[k], 0), np.int64)
+    else:
+        O = oracle.Oracle()
+        ix = O.build_index(s, e)
+        kind, desc = "port", "oracle restatement of search_idxs"
+        run = lambda: O.search_batch(ix, sq, se, mode=1, threads=threads, lists=False)
+        lists = lambda k: O.search(ix, sq[k], se[k], 0).astype(np.int64)
+    build_s = time.perf_counter() - t0
+    ts = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        run()
+        ts.append(time.perf_counter() - t1)
+    t = sorted(ts)[1]
+    bad = sum(int(not np.array_equal(gpu_lists[int(q)], lists(k))) for k, q in enumerate(picks))
+    return {"value": len(picks) / t, "unit": "queries/s", "cores": threads, "kind": kind, "cpu_model": cpu_model(),
+            "sample": f"{len(picks)} queries spread evenly over the {qs.size}, median of 3; {desc}; "
+                      f"static chunks over {threads} threads (build {build_s:.1f} s, not timed)",
+            "checked_queries": int(len(picks)), "mismatched_queries": int(bad)}
+
+
 def verify_output(ix, tqs, tqe, s, e, off, idx, nq, stream):
     """Checks the timed configuration's own output on the device: counts against the closed
     form (torch.searchsorted over the starts and the sorted ends, every query has s <= e), the
@@ -417,7 +453,6 @@ def bench_skewed(a):
     gen_s = time.perf_counter() - t0
     print("[skewed] generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
     ix = sia.DeviceIndex(np.int64).build(s, e)
-    del s, e
     tqs, tqe = torch.from_numpy(qs).to(dev), torch.from_numpy(qe).to(dev)
     stream = torch.cuda.current_stream()
     plan = torch.empty(ix.plan_bytes(nq), dtype=torch.uint8, device=dev)
@@ -496,6 +531,11 @@ def bench_skewed(a):
           % (ends32_ms[1], ends32_ms[0]), file=sys.stderr, flush=True)
     t_fill = 0.0
     tot = 0
+    # a sample of the queries (spread over all batches) is kept for the oracle check below,
+    # and every batch's counts are compared with the walk's of the whole-batch count
+    sample = np.unique(np.linspace(0, nq - 1, a.skewed_check).astype(np.int64)) if a.skewed_check else np.zeros(0, np.int64)
+    kept = {}
+    counts_equal = True
     t0 = time.perf_counter()
     for b in range(nb):
         lo, hi = b * bq, min(nq, (b + 1) * bq)
@@ -516,9 +556,22 @@ def bench_skewed(a):
         torch.cuda.synchronize()
         t_fill += e0.elapsed_time(e1)
         tot += t
+        counts_equal = counts_equal and bool(torch.equal((off[1:m + 1] - off[:m]).to(torch.int32), c_walk[lo:hi]))
+        for q in sample[(sample >= lo) & (sample < hi)]:
+            o0, o1 = int(off[q - lo].item()), int(off[q - lo + 1].item())
+            kept[int(q)] = idx[o0:o1].cpu().numpy().astype(np.int64)
     wall = time.perf_counter() - t0
     autotune.apply(saved)
     assert tot == hits, (tot, hits)
+    verified = {"streamed_total_equals_count": tot == hits, "batch_offsets_equal_walk_counts": counts_equal}
+    cpu = None
+    if len(sample) and not a.no_cpu:
+        cpu = cpu_baseline_sampled(s, e, qs, qe, sample, kept)
+        verified.update({"cpu_sample_queries": cpu["checked_queries"],
+                         "cpu_sample_lists_equal": cpu["mismatched_queries"] == 0})
+        print("[skewed] CPU check of %d sampled queries: %d mismatched" % (cpu["checked_queries"],
+              cpu["mismatched_queries"]), file=sys.stderr, flush=True)
+    del s, e
     print("[skewed] streamed search: %d batches, %.3g positions, fill %.1f ms, wall %.2f s" % (nb, tot, t_fill, wall),
           file=sys.stderr, flush=True)
     # roofline of the streamed fill: the bytes it must move (positions written, plan entry and
@@ -554,8 +607,11 @@ def bench_skewed(a):
            "fill_batch_ms_ends32": ends32_ms.get(1), "fill_batch_ms_int64_ends": ends32_ms.get(0),
            "search": {"batches": nb, "fill_ms_total": t_fill, "positions_per_s": tot / (t_fill / 1e3),
                       "wall_s_including_plan_offsets_sync": wall},
-           "roofline": roof}
+           "roofline": roof, "cpu_baseline": cpu, "verified": verified}
     print(json.dumps(out))
+    if not all(v for k, v in verified.items() if isinstance(v, bool)):
+        print("[skewed] VERIFICATION FAILED: %s" % verified, file=sys.stderr, flush=True)
+        sys.exit(3)
 
 
 def bench_graph(a):
