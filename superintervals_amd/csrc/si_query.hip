@@ -526,7 +526,7 @@ __device__ __forceinline__ void emit_segments(const Seg (&sg_in)[N], int lane, E
         uint32_t pre = pre0;
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-            if (sg[j].c != 0 && pre >= x0 && pre - x0 < (uint32_t)WAVE)
+            if (sg[j].c != 0 && pre - x0 < (uint32_t)WAVE)   // (pre < x0 wraps to a large value)
                 put_segment(sd, pre - x0, pre, sg[j].c, sg[j].mk, sg[j].base, sg[j].o);
             pre += sg[j].c;
         }
@@ -543,13 +543,10 @@ __device__ __forceinline__ void emit_segments(const Seg (&sg_in)[N], int lane, E
         const uint32_t r = x - d.pre;
         const uint32_t om = d.kv & 0xFFFFu;
         uint32_t pos;
-        if ((d.kv >> 16) & 1u) {
-            pos = d.base + r;
-        } else if ((d.kv >> 17) & 1u) {
-            pos = d.base - r;
-        } else if (SI_EMIT_LUT) {
-            // the (r+1)-th highest set bit of om, from the byte table (r >= c only happens
-            // on slots that store nothing)
+        if (SI_EMIT_LUT) {
+            // both forms computed and one selected (no branch): a range's value, or the
+            // (r+1)-th highest set bit of om from the byte table (for a range the table read
+            // is in bounds and unused; r >= c only happens on slots that store nothing)
             uint32_t byte = om & 0xFFu, k = r, add = 0;
             if (W == 16) {
                 const uint32_t hi = om >> 8;
@@ -559,7 +556,13 @@ __device__ __forceinline__ void emit_segments(const Seg (&sg_in)[N], int lane, E
                 k = up ? r : r - ph;
                 add = up ? 8u : 0u;
             }
-            pos = d.base + add + ((kbit[byte] >> ((4u * k) & 31u)) & 7u);
+            const uint32_t mpos = d.base + add + ((kbit[byte] >> ((4u * k) & 31u)) & 7u);
+            const uint32_t rpos = ((d.kv >> 16) & 1u) ? d.base + r : d.base - r;
+            pos = ((d.kv >> 16) & 3u) ? rpos : mpos;
+        } else if ((d.kv >> 16) & 1u) {
+            pos = d.base + r;
+        } else if ((d.kv >> 17) & 1u) {
+            pos = d.base - r;
         } else {
             // the (r+1)-th highest set bit of om: largest p with popc(om >> p) >= r + 1
             uint32_t p = 0;
