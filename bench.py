@@ -932,6 +932,15 @@ def main():
     torch.cuda.synchronize()
     autotune.apply({"count_rank": tuned_count_rank})
     t_count_rank = c0.elapsed_time(c1) / a.steps
+    # count() from the queries, the whole call (plan + the counts it keeps, scattered)
+    ix.count_device(tqs, tqe, counts, stream=stream)
+    c0.record(stream)
+    for _ in range(a.steps):
+        ix.count_device(tqs, tqe, counts, stream=stream)
+    c1.record(stream)
+    torch.cuda.synchronize()
+    t_count_call = c0.elapsed_time(c1) / a.steps
+    checks["count_call_equals_offsets"] = bool(torch.equal(counts.to(torch.int64), off[1:] - off[:-1]))
 
     # Algorithmic bytes (per query: c = coordinate bytes, V/J = the reference walk's ends
     # reads and branch jumps, h = hits).  Each launch is credited with its own work only:
@@ -946,6 +955,8 @@ def main():
     kernels = {
         "count": {"ms": t_count, "alg_bytes": b_count_walk, "GBps": b_count_walk / t_count / 1e6},
         "count_rank": {"ms": t_count_rank, "queries_per_s": nq / (t_count_rank / 1e3)},
+        "count_call": {"ms": t_count_call, "queries_per_s": nq / (t_count_call / 1e3),
+                       "note": "count() from the queries, the whole call: plan (sort, bounds, rank counts) + scatter"},
         "plan": {"ms": t_plan},
         "search_offsets": {"ms": t_off},
         "search_fill": {"ms": t_fill, "alg_bytes": b_fill, "GBps": b_fill / t_fill / 1e6},
