@@ -100,6 +100,9 @@ def tune(ix, plan_t, nq, offsets_t, idx_t, stream=None, reps=3, tiles=CANDIDATE_
     fill = lambda: ix.fill_planned(plan_t, nq, offsets_t, idx_t, stream=stream)
     results = []
     try:
+        # the tile and the workgroup counts are judged on the walk form of count (the rank
+        # form reads neither); the form itself is chosen afterwards
+        apply({"count_rank": 0})
         for tile in tiles:
             for cw in wgs:
                 for fw in wgs:
