@@ -457,12 +457,6 @@ def bench_skewed(a):
     stream = torch.cuda.current_stream()
     plan = torch.empty(ix.plan_bytes(nq), dtype=torch.uint8, device=dev)
     counts = torch.empty(nq, dtype=torch.int32, device=dev)
-    ev = events(2)
-    ev[0].record(stream)
-    ix.plan_device(tqs, tqe, plan, stream=stream)
-    ev[1].record(stream)
-    torch.cuda.synchronize()
-    t_plan = ev[0].elapsed_time(ev[1])
     saved = autotune.current()
 
     def timed(fn, reps=3):
@@ -476,6 +470,8 @@ def bench_skewed(a):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps
 
+    # the plan of all 20M queries (sort by end, bounds and rank counts), warm
+    t_plan = timed(lambda: ix.plan_device(tqs, tqe, plan, stream=stream))
     try:
         autotune.apply({"count_rank": 0, "heavy_ends32": 0})
         t_walk64 = timed(lambda: ix.count_planned(plan, nq, counts, stream=stream))
@@ -485,12 +481,17 @@ def bench_skewed(a):
         autotune.apply({"count_rank": 1})
         t_rank = timed(lambda: ix.count_planned(plan, nq, counts, stream=stream))
         same = bool(torch.equal(counts, c_walk))
+        # count() from the queries, the whole call (plan with its rank counts + scatter)
+        counts.zero_()
+        t_call = timed(lambda: ix.count_device(tqs, tqe, counts, stream=stream))
+        same_call = bool(torch.equal(counts, c_walk))
     finally:
         autotune.apply(saved)
     hits = int(c_walk.to(torch.int64).sum().item())
     wide = int((c_walk > 100_000).sum().item())
-    print("[skewed] count: walk %.2f ms (int64 ends in the wave walk: %.2f ms), rank %.2f ms, identical %s, %.3g hits"
-          % (t_walk, t_walk64, t_rank, same, hits), file=sys.stderr, flush=True)
+    print("[skewed] count: walk %.2f ms (int64 ends in the wave walk: %.2f ms), rank counts of the plan %.2f ms, "
+          "count() whole call %.2f ms, identical %s/%s, %.3g hits"
+          % (t_walk, t_walk64, t_rank, t_call, same, same_call, hits), file=sys.stderr, flush=True)
     # streamed search: batches of bq queries through plan -> offsets -> fill (descending)
     nb = (nq + bq - 1) // bq
     bplan = torch.empty(ix.plan_bytes(bq), dtype=torch.uint8, device=dev)
@@ -563,7 +564,8 @@ def bench_skewed(a):
     wall = time.perf_counter() - t0
     autotune.apply(saved)
     assert tot == hits, (tot, hits)
-    verified = {"streamed_total_equals_count": tot == hits, "batch_offsets_equal_walk_counts": counts_equal}
+    verified = {"streamed_total_equals_count": tot == hits, "batch_offsets_equal_walk_counts": counts_equal,
+                "rank_counts_equal_walk": same, "count_call_equals_walk": same_call}
     cpu = None
     if len(sample) and not a.no_cpu:
         cpu = cpu_baseline_sampled(s, e, qs, qe, sample, kept)
@@ -600,7 +602,10 @@ def bench_skewed(a):
                       "intervals": n, "queries": nq, "search_batch": bq, "host_generation_s": round(gen_s, 2),
                       "build_ms": round(ix.last_build_ms, 2)},
            "plan_ms": t_plan, "count_walk_ms": t_walk, "count_walk_int64_ends_ms": t_walk64,
-           "count_rank_ms": t_rank, "count_identical": same,
+           "count_rank_ms": t_rank, "count_identical": same, "count_call_ms": t_call,
+           "count_note": "plan_ms: sort by end, bounds and rank-form counts of all queries (warm); count_rank_ms: "
+                         "the scatter of the plan's counts; count_call_ms: count() from the queries (plan + scatter); "
+                         "count_walk_ms: the walk form on the planned batch",
            "hits": hits, "queries_over_1e5_hits": wide,
            "lane_steps_sweep_ms": {str(k): v for k, v in sweep.items()}, "lane_steps": best_ls,
            "lane_steps_count_walk_ms": {str(k): v for k, v in csweep.items()},

@@ -102,7 +102,10 @@ class DeviceIndex:
             self._h = None
 
     def __del__(self):
-        self.close()
+        # at interpreter exit the module's globals may already be cleared; the process
+        # releases its device memory then anyway
+        if N is not None and getattr(N, "lib", None) is not None:
+            self.close()
 
     @property
     def handle(self) -> int:
@@ -385,7 +388,8 @@ class SetTable:
         self.slots = []
 
     def __del__(self):
-        self.close()
+        if N is not None and getattr(N, "lib", None) is not None:   # (cleared at interpreter exit)
+            self.close()
 
     def put(self, slot: int, index):
         """Put a built int32 DeviceIndex into slot (None empties it)."""
@@ -445,7 +449,8 @@ class LearnedIndex:
             self._h = None
 
     def __del__(self):
-        self.close()
+        if N is not None and getattr(N, "lib", None) is not None:   # (cleared at interpreter exit)
+            self.close()
 
     def train(self, steps: int = 400, batch: int = 65536, lr: float = 3e-3, stream=None):
         N.check(N.lib.siLearnedTrain(self._h, steps, batch, lr, _stream_of(stream)), "siLearnedTrain")

@@ -475,14 +475,22 @@ struct Seg {
 constexpr int EMIT_EVERY = SI_EMIT_EVERY;
 static_assert(EMIT_EVERY >= 2 && EMIT_EVERY <= 4, "2 to 4 segments per emission");
 
-// Range segments longer than this are written on their own (see emit_segments).
+// Range segments of at least WAVE_RANGE values are written on their own, one at a time by
+// the whole wave (see emit_segments): a few instructions per 64 values there, against the
+// slot search and both position forms per slot in the shared chunks.
+#ifndef SI_WAVE_RANGE
+#define SI_WAVE_RANGE 64
+#endif
+constexpr uint32_t WAVE_RANGE = SI_WAVE_RANGE;
 constexpr uint32_t BIG_SEG = 1u << 23;
+static_assert(WAVE_RANGE >= 1 && WAVE_RANGE <= BIG_SEG, "ranges longer than BIG_SEG always go on their own");
 static_assert((uint64_t)4 * 64 * BIG_SEG <= (1ull << 31), "the slots left for the scan fit in 32 bits");
 
 // Every lane holds up to N segments; they occupy consecutive slots of the concatenation.
-// Range segments longer than BIG_SEG (a run of millions of hits) are first written one at a
-// time by the whole wave, so that the slots of the rest, at most 64 x N x BIG_SEG, stay
-// within the 32-bit scan.
+// Range segments of at least WAVE_RANGE values are first written one at a time by the whole
+// wave; what is left for the shared chunks are line masks (at most W values each) and ranges
+// shorter than WAVE_RANGE <= BIG_SEG, so its slots, at most 64 x N x BIG_SEG, stay within the
+// 32-bit scan.
 template <typename S, int MODE, int N>
 __device__ __forceinline__ void emit_segments(const Seg (&sg_in)[N], int lane, EmitSlot* __restrict__ sd,
                                               const uint32_t* __restrict__ kbit,
@@ -495,7 +503,7 @@ __device__ __forceinline__ void emit_segments(const Seg (&sg_in)[N], int lane, E
     for (int j = 0; j < N; ++j) sg[j] = sg_in[j];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        uint64_t big = __ballot(sg[j].c > BIG_SEG);
+        uint64_t big = __builtin_amdgcn_ballot_w64(((sg[j].mk >> 16) & 3u) != 0u && sg[j].c >= WAVE_RANGE);
         while (big) {
             const int L = __builtin_ctzll(big);
             big &= big - 1ull;
