@@ -433,6 +433,38 @@ def test_wide_queries_guaranteed_run(sia, orc, dtype):
     ix.close()
 
 
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_range_lengths_around_wave_written_threshold(sia, orc, dtype):
+    """Range segments of 64 or more values are written by the whole wave, shorter ones in
+    the shared 64-slot chunks.  Runs of exactly 1..1000 values (63, 64, 65, 127, 128, 129
+    among them), next to line masks from a few longer intervals, in every mode and order,
+    against the oracle."""
+    print(f"[gpu] range lengths around the wave-written threshold dtype={np.dtype(dtype).name}", flush=True)
+    rng = np.random.default_rng(5)
+    n = 100_000
+    s = np.arange(n, dtype=np.int64)
+    e = s + 1
+    longer = rng.random(n) < 0.05
+    e = np.where(longer, s + rng.integers(2, 300, n), e)
+    lens = np.array([1, 2, 31, 32, 33, 63, 64, 65, 66, 127, 128, 129, 191, 192, 193, 1000])
+    L = np.repeat(lens, 300)
+    qs = rng.integers(0, n - 1200, L.size)
+    qe = qs + L - 1
+    base = (np.int64(5) << 33) if dtype == np.int64 else 0
+    s, e, qs, qe = ((x + base).astype(dtype) for x in (s, e, qs, qe))
+    ix = _dev_index(sia, s, e, np.arange(n, dtype=np.int32))
+    oix = orc.build_index(s, e)
+    assert np.array_equal(ix.count(qs, qe), orc.count_batch(oix, qs, qe))
+    for order, mode in (("desc", 0), ("vector", 1)):
+        off, idx = ix.search(qs, qe, order=order)
+        ooff, oidx = orc.search_batch(oix, qs, qe, mode=mode)
+        assert np.array_equal(off, ooff) and np.array_equal(idx.astype(np.int64), oidx), order
+    off, vals = ix.search(qs, qe, values=True)
+    ooff, oidx = orc.search_batch(oix, qs, qe, mode=0)
+    assert np.array_equal(vals, oix["data"][oidx])
+    ix.close()
+
+
 @pytest.mark.parametrize("dtype,long_len", [(np.int64, 0), (np.int64, 4096), (np.int32, 64), (np.int32, 1)])
 def test_long_list_walk(sia, orc, dtype, long_len):
     """The wave walk with a long list: hits below the first start >= s - len come from the
