@@ -408,12 +408,16 @@ __device__ __forceinline__ uint32_t miss_top(uint32_t kc, uint32_t m) {
 }
 
 // One slot of a wave's emission scratch in LDS: the segment that starts at this output
-// slot of the current 64-slot chunk.  kv = m | kind << 16 | 1 << 31 (valid); 0 when free.
-// kind bit 0: ascending range base .. base + c - 1; bit 1: descending range base ..
-// base - c + 1; neither: the set bits of m, highest first, offset by base.
-struct __align__(8) EmitSlot {
-    uint32_t pre, kv, base, c;
-    uint32_t o_lo, o_hi;
+// slot of the current 64-slot chunk, 16 bytes (one LDS access each way).  kv = m | dir << 16
+// | pre << 18, never 0 for a segment (a mask segment has m != 0, a range dir != 0); 0 when
+// the slot is free.  dir (a signed 2-bit field): +1 for the ascending range base ..
+// base + c - 1, -1 for the descending range base .. base - c + 1, 0 for the set bits of m,
+// highest first, offset by base (a range's m is 0).  pre: the segment's first slot in the
+// emission's concatenation (< 2^14).  a_lo / a_hi: the address of the segment's first
+// output element (so a slot's store address is one add away).
+struct __align__(16) EmitSlot {
+    uint32_t kv, base;
+    uint32_t a_lo, a_hi;
 };
 
 // Orders this wave's LDS accesses for the compiler; LDS executes a wave's accesses in
@@ -427,8 +431,9 @@ __device__ __forceinline__ void wave_lds_sync() { __builtin_amdgcn_fence(__ATOMI
 // chunk goes through the wave's 64 LDS slots (sd): a lane whose segment starts in the
 // chunk writes its descriptor at its start slot, a ballot gives the chunk's start mask,
 // and slot x takes the last start at or before x (or the segment running in from the
-// previous chunk).  A store happens only at r < c of its segment, so no slot is written
-// outside the segment its owner was given.  All lanes execute the ballots.
+// previous chunk).  The segments are contiguous in the concatenation, so every slot
+// below the total lies in the segment found for it: no slot is written outside the
+// segment its owner was given.  All lanes execute the ballots.
 #ifndef SI_EMIT_LUT
 #define SI_EMIT_LUT 1
 #endif
@@ -446,18 +451,9 @@ __device__ __forceinline__ uint32_t kbit_entry(uint32_t b) {
     return e;
 }
 
-// Segment descriptor written at its start slot of the current chunk.
-__device__ __forceinline__ void put_segment(EmitSlot* __restrict__ sd, uint32_t slot, uint32_t pre, uint32_t c,
-                                            uint32_t mk, uint32_t base, uint64_t o) {
-    EmitSlot e;
-    e.pre = pre;
-    e.kv = mk | 0x80000000u;
-    e.base = base;
-    e.c = c;
-    e.o_lo = (uint32_t)o;
-    e.o_hi = (uint32_t)(o >> 32);
-    sd[slot] = e;
-}
+// A 32-bit word in global memory: output addresses rebuilt from integers keep their address
+// space, so their stores stay global stores (a generic pointer would make them flat ones).
+typedef __attribute__((address_space(1))) uint32_t global_u32;
 
 // One output segment of a lane: c values at output offset o, either the ascending range
 // base .. base + c - 1 (asc), the descending range base .. base - c + 1 (desc), or the set
@@ -466,6 +462,20 @@ struct Seg {
     uint32_t c, mk, base;   // mk = m | asc << 16 | desc << 17
     uint64_t o;
 };
+
+// Segment descriptor written at its start slot of the current chunk (a: the address of the
+// segment's first output element): the kind bits of mk become the signed step dir
+// (asc 01 -> +1, desc 10 -> 11 = -1, mask 00 -> 0).
+__device__ __forceinline__ void put_segment(EmitSlot* __restrict__ sd, uint32_t slot, uint32_t pre, uint32_t mk,
+                                            uint32_t base, uint64_t a) {
+    const uint32_t kind = (mk >> 16) & 3u;
+    EmitSlot e;
+    e.kv = (mk & 0xFFFFu) | (kind | (kind >> 1)) << 16 | pre << 18;
+    e.base = base;
+    e.a_lo = (uint32_t)a;
+    e.a_hi = (uint32_t)(a >> 32);
+    sd[slot] = e;
+}
 
 // Steps whose segments the fill walk holds and emits together (their segments of one
 // query are adjacent in its output list, so each emission writes longer runs).
@@ -477,20 +487,20 @@ static_assert(EMIT_EVERY >= 2 && EMIT_EVERY <= 4, "2 to 4 segments per emission"
 
 // Range segments of at least WAVE_RANGE values are written on their own, one at a time by
 // the whole wave (see emit_segments): a few instructions per 64 values there, against the
-// slot search and both position forms per slot in the shared chunks.
+// slot search and the position per slot in the shared chunks.  At most 64, so that every
+// segment left for the chunks is shorter than a chunk (at most 16 values for a line mask).
 #ifndef SI_WAVE_RANGE
 #define SI_WAVE_RANGE 64
 #endif
 constexpr uint32_t WAVE_RANGE = SI_WAVE_RANGE;
-constexpr uint32_t BIG_SEG = 1u << 23;
-static_assert(WAVE_RANGE >= 1 && WAVE_RANGE <= BIG_SEG, "ranges longer than BIG_SEG always go on their own");
-static_assert((uint64_t)4 * 64 * BIG_SEG <= (1ull << 31), "the slots left for the scan fit in 32 bits");
+static_assert(WAVE_RANGE >= 16 && WAVE_RANGE <= 64, "chunk segments are shorter than a chunk");
+// the concatenation of one emission's chunk segments fits EmitSlot's 14-bit pre
+static_assert(64 * (EMIT_EVERY + 1) * (WAVE_RANGE - 1) < (1 << 14), "emission slots fit 14 bits");
 
 // Every lane holds up to N segments; they occupy consecutive slots of the concatenation.
 // Range segments of at least WAVE_RANGE values are first written one at a time by the whole
 // wave; what is left for the shared chunks are line masks (at most W values each) and ranges
-// shorter than WAVE_RANGE <= BIG_SEG, so its slots, at most 64 x N x BIG_SEG, stay within the
-// 32-bit scan.
+// shorter than WAVE_RANGE, at most 64 x N x 64 slots.
 template <typename S, int MODE, int N>
 __device__ __forceinline__ void emit_segments(const Seg (&sg_in)[N], int lane, EmitSlot* __restrict__ sd,
                                               const uint32_t* __restrict__ kbit,
@@ -498,6 +508,8 @@ __device__ __forceinline__ void emit_segments(const Seg (&sg_in)[N], int lane, E
                                               const int32_t* __restrict__ vals,
                                               int32_t* __restrict__ out_val) {
     constexpr int W = Line<S>::W;
+    // the output array, as 32-bit words (positions, or values)
+    uint32_t* const dst = (MODE == W_VALUES) ? reinterpret_cast<uint32_t*>(out_val) : out_idx;
     Seg sg[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) sg[j] = sg_in[j];
@@ -514,10 +526,15 @@ __device__ __forceinline__ void emit_segments(const Seg (&sg_in)[N], int lane, E
             const uint64_t bo = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((uint32_t)(sg[j].o >> 32), L) << 32) |
                                 (uint64_t)(uint32_t)__builtin_amdgcn_readlane((uint32_t)sg[j].o, L);
             const bool asc = (bmk >> 16) & 1u;
-            for (uint32_t x = (uint32_t)lane; x < bc; x += WAVE) {
-                const uint32_t p = asc ? bbase + x : bbase - x;
-                if (MODE == W_VALUES) out_val[bo + x] = vals[p];
-                else out_idx[bo + x] = p;
+            // rounds of 64 values (counted so that x never wraps, whatever bc is)
+            const uint32_t rounds = (uint32_t)(((uint64_t)bc + WAVE - 1u) / WAVE);
+            for (uint32_t t = 0; t < rounds; ++t) {
+                const uint32_t x = t * (uint32_t)WAVE + (uint32_t)lane;
+                if (x < bc) {
+                    const uint32_t p = asc ? bbase + x : bbase - x;
+                    if (MODE == W_VALUES) out_val[bo + x] = vals[p];
+                    else out_idx[bo + x] = p;
+                }
             }
             if (lane == L) sg[j].c = 0u;
         }
@@ -535,11 +552,11 @@ __device__ __forceinline__ void emit_segments(const Seg (&sg_in)[N], int lane, E
 #pragma unroll
         for (int j = 0; j < N; ++j) {
             if (sg[j].c != 0 && pre - x0 < (uint32_t)WAVE)   // (pre < x0 wraps to a large value)
-                put_segment(sd, pre - x0, pre, sg[j].c, sg[j].mk, sg[j].base, sg[j].o);
+                put_segment(sd, pre - x0, pre, sg[j].mk, sg[j].base, (uint64_t)(uintptr_t)(dst + sg[j].o));
             pre += sg[j].c;
         }
         wave_lds_sync();
-        const uint64_t mine = __ballot((sd[lane].kv >> 31) != 0u) & le;
+        const uint64_t mine = __builtin_amdgcn_ballot_w64(sd[lane].kv != 0u) & le;
         // slot WAVE holds the segment running in from the previous chunk (never read in
         // the first chunk, whose slot 0 always starts a segment)
         const EmitSlot d = sd[mine ? 63u - (uint32_t)__builtin_clzll(mine) : (uint32_t)WAVE];
@@ -548,13 +565,14 @@ __device__ __forceinline__ void emit_segments(const Seg (&sg_in)[N], int lane, E
         // the segment holding the chunk's last slot runs into the next chunk
         if (lane == WAVE - 1) sd[WAVE] = d;
         const uint32_t x = x0 + (uint32_t)lane;
-        const uint32_t r = x - d.pre;
+        const uint32_t r = x - (d.kv >> 18);
         const uint32_t om = d.kv & 0xFFFFu;
-        uint32_t pos;
+        // The mask form's position (for a range, m = 0 gives base), plus r * dir for a
+        // range: one expression, no branch or select.  (A chunk segment spans at most two
+        // chunks, so a stored slot has r < 128, and r * dir fits the 24-bit multiply.)
+        uint32_t mpos;
         if (SI_EMIT_LUT) {
-            // both forms computed and one selected (no branch): a range's value, or the
-            // (r+1)-th highest set bit of om from the byte table (for a range the table read
-            // is in bounds and unused; r >= c only happens on slots that store nothing)
+            // the (r+1)-th highest set bit of om from the byte table
             uint32_t byte = om & 0xFFu, k = r, add = 0;
             if (W == 16) {
                 const uint32_t hi = om >> 8;
@@ -564,47 +582,20 @@ __device__ __forceinline__ void emit_segments(const Seg (&sg_in)[N], int lane, E
                 k = up ? r : r - ph;
                 add = up ? 8u : 0u;
             }
-            const uint32_t mpos = d.base + add + ((kbit[byte] >> ((4u * k) & 31u)) & 7u);
-            const uint32_t rpos = ((d.kv >> 16) & 1u) ? d.base + r : d.base - r;
-            pos = ((d.kv >> 16) & 3u) ? rpos : mpos;
-        } else if ((d.kv >> 16) & 1u) {
-            pos = d.base + r;
-        } else if ((d.kv >> 17) & 1u) {
-            pos = d.base - r;
+            mpos = d.base + add + ((kbit[byte] >> ((4u * k) & 31u)) & 7u);
         } else {
             // the (r+1)-th highest set bit of om: largest p with popc(om >> p) >= r + 1
             uint32_t p = 0;
 #pragma unroll
             for (int sh = W / 2; sh >= 1; sh >>= 1)
                 p = ((uint32_t)__popc(om >> (p + sh)) >= r + 1u) ? p + sh : p;
-            pos = d.base + p;
+            mpos = d.base + p;
         }
-        if (x < total && r < d.c) {
-            const uint64_t oo = ((uint64_t)d.o_hi << 32) | d.o_lo;
-            if (MODE == W_VALUES) out_val[oo + r] = vals[pos];
-            else out_idx[oo + r] = pos;
-        }
-        // The segment holding the chunk's last slot runs on into the next chunk.  When it is a
-        // range that covers whole chunks beyond this one, no other segment starts in them:
-        // they are written here with plain stores, and the loop goes on at the first chunk
-        // the range does not cover (which it still enters, from slot WAVE).
-        // (readlane returns int: every word is taken as uint32_t before it is widened)
-        const uint32_t lkv = (uint32_t)__builtin_amdgcn_readlane(d.kv, WAVE - 1);
-        if ((lkv >> 16) & 3u) {
-            const uint32_t lpre = (uint32_t)__builtin_amdgcn_readlane(d.pre, WAVE - 1);
-            const uint32_t lc = (uint32_t)__builtin_amdgcn_readlane(d.c, WAVE - 1);
-            const uint32_t lbase = (uint32_t)__builtin_amdgcn_readlane(d.base, WAVE - 1);
-            const uint64_t lo = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(d.o_hi, WAVE - 1) << 32) |
-                                (uint64_t)(uint32_t)__builtin_amdgcn_readlane(d.o_lo, WAVE - 1);
-            const uint32_t lend = min(lpre + lc, total);   // the range's end in the concatenation
-            const bool asc = (lkv >> 16) & 1u;
-            while (x0 + 2u * WAVE <= lend) {
-                x0 += WAVE;
-                const uint32_t rr = x0 + (uint32_t)lane - lpre;
-                const uint32_t p = asc ? lbase + rr : lbase - rr;
-                if (MODE == W_VALUES) out_val[lo + rr] = vals[p];
-                else out_idx[lo + rr] = p;
-            }
+        const int32_t dir = __builtin_amdgcn_sbfe((int32_t)d.kv, 16, 2);
+        const uint32_t pos = (uint32_t)__mul24((int32_t)(r & 127u), dir) + mpos;
+        if (x < total) {
+            global_u32* const a = reinterpret_cast<global_u32*>((uintptr_t)(((uint64_t)d.a_hi << 32) | d.a_lo));
+            a[r] = (MODE == W_VALUES) ? (uint32_t)vals[pos] : pos;
         }
     }
 }
@@ -1602,7 +1593,7 @@ __global__ __launch_bounds__(PB_THREADS) void k_plan_bounds_tree(IndexView<S> ix
         }
         const uint32_t ub = (j[0] < 0) ? NONE : (uint32_t)j[0];
         pv.idx[k] = ub;
-        if (LB) pv.lb[k] = (!s1 || j[1] < 0) ? 0u : (uint32_t)j[1] + 1u;
+        if (LB) pv.lb[k] = (!s1 || j[LB ? 1 : 0] < 0) ? 0u : (uint32_t)j[LB ? 1 : 0] + 1u;
         if (RANK) {
             const int64_t jr = j[N - 1];
             uint32_t c = 0;
