@@ -1393,33 +1393,68 @@ __global__ __launch_bounds__(Q_THREADS) void k_upper_bound(IndexView<S> ix, cons
     if (q < n) out[q] = upper_bound_pos(ix.starts, ix.n, xs[q]);
 }
 
-// Last r in [lo, hi] with v[r] <= x, or lo - 1 (branchless halving).
-template <typename S>
-__device__ __forceinline__ int64_t last_le_in(const S* __restrict__ v, int64_t lo, int64_t hi, S x) {
-    const S* base = v + lo;
-    uint32_t len = (uint32_t)(hi - lo + 1);
-    while (len > 1) {
-        const uint32_t half = len >> 1;
-        base = (base[half] <= x) ? base + half : base;
-        len -= half;
+// Ranks of N keys in Eytzinger tops of h levels (LDS), together: the N reads of a level are
+// issued at once.  Each descent ends at k = 2^h + r, r = the number of top keys <= x[i];
+// padding keys are max(S) and a key equal to max(S) may count padding, so r is clamped to
+// the level's size.
+template <typename S, int N>
+__device__ __forceinline__ void top_ranks(const S* (&tp)[N], uint32_t h, const uint32_t (&size)[N], const S (&x)[N],
+                                          uint32_t (&r)[N]) {
+    uint32_t k[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) k[i] = 1u;
+    for (uint32_t lvl = 0; lvl < h; ++lvl) {
+        S t[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) t[i] = tp[i][k[i] - 1u];
+#pragma unroll
+        for (int i = 0; i < N; ++i) k[i] = 2u * k[i] + (t[i] <= x[i] ? 1u : 0u);
     }
-    return (int64_t)(base - v) - ((*base <= x) ? 0 : 1);
+#pragma unroll
+    for (int i = 0; i < N; ++i) r[i] = min(k[i] - (1u << h), size[i]);
+}
+
+// Last position p in [b[i], hi[i]] (one node: at most 64 keys) with v[i][p] <= x[i], or
+// b[i] - 1, for N searches together: fixed halving steps 32, 16, .., 1, so the N probes of
+// a step are issued at once, and the node's first key is read beside the first step.
+template <typename S, int N>
+__device__ __forceinline__ void node_last_le(const S* (&v)[N], const uint32_t (&b)[N], const uint32_t (&hi)[N],
+                                             const S (&x)[N], int64_t (&out)[N]) {
+    uint32_t p[N];
+    bool first_le[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        p[i] = b[i];
+        first_le[i] = v[i][b[i]] <= x[i];
+    }
+#pragma unroll
+    for (uint32_t step = 32u; step >= 1u; step >>= 1) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const uint32_t q = p[i] + step;
+            if (q <= hi[i] && v[i][q] <= x[i]) p[i] = q;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = (p[i] == b[i] && !first_le[i]) ? (int64_t)b[i] - 1 : (int64_t)p[i];
 }
 
 // Last position with key <= x, or NONE, through the tree (s_top = its top level in LDS).
 template <typename S>
 __device__ __forceinline__ uint32_t ub_tree_pos(const UbTree<S>& t, const S* s_top, uint32_t h, S x) {
-    uint32_t k = 1, r = 0;
-    for (uint32_t lvl = 0; lvl < h; ++lvl) {
-        const bool right = s_top[k - 1] <= x;
-        r += right ? (1u << (h - 1u - lvl)) : 0u;   // the left subtree and the node itself
-        k = 2u * k + (right ? 1u : 0u);
-    }
-    // padding keys are max(S); a key equal to max(S) may count padding, so clamp
-    int64_t j = (int64_t)std::min<uint32_t>(r, t.size[t.nlv]) - 1;
+    const S* tp[1] = {s_top};
+    const uint32_t tsz[1] = {t.size[t.nlv]};
+    const S xx[1] = {x};
+    uint32_t r[1];
+    top_ranks<S, 1>(tp, h, tsz, xx, r);
+    int64_t j = (int64_t)r[0] - 1;
     for (int l = t.nlv - 1; l >= 0 && j >= 0; --l) {
-        const int64_t b = j * 64, e = std::min<int64_t>(b + 63, (int64_t)t.size[l] - 1);
-        j = last_le_in(t.lv[l], b, e, x);
+        const S* v[1] = {t.lv[l]};
+        const uint32_t b[1] = {(uint32_t)j * 64u};
+        const uint32_t hi[1] = {min(b[0] + 63u, t.size[l] - 1u)};
+        int64_t o[1];
+        node_last_le<S, 1>(v, b, hi, xx, o);
+        j = o[0];
     }
     return (j < 0) ? NONE : (uint32_t)j;
 }
@@ -1519,19 +1554,6 @@ int launch_ub_tree(const siIndex* ix, const S* xs, size_t n, uint32_t* out, hipS
     return SI_OK;
 }
 
-// Number of keys <= x of a tree's top level (Eytzinger order in LDS, h levels), clamped
-// to the level's size (padding keys are max(S)).
-template <typename S>
-__device__ __forceinline__ uint32_t top_rank(const S* s_top, uint32_t h, uint32_t size, S x) {
-    uint32_t k = 1, r = 0;
-    for (uint32_t lvl = 0; lvl < h; ++lvl) {
-        const bool right = s_top[k - 1] <= x;
-        r += right ? (1u << (h - 1u - lvl)) : 0u;
-        k = 2u * k + (right ? 1u : 0u);
-    }
-    return min(r, size);
-}
-
 // The plan's bounds and counts: idx[k] = upper_bound(e[k]); with LB, lb[k] = the first
 // position whose start is >= s[k] (upper_bound(s - 1) + 1, 0 when s is the smallest value);
 // with RANK, cnt[k] = the rank-form count (upper_bound + 1 - #{ends < s}, #{ends < s} =
@@ -1574,22 +1596,25 @@ __global__ __launch_bounds__(PB_THREADS) void k_plan_bounds_tree(IndexView<S> ix
         tr[n] = &ts; tp[n] = s_ts; x[n] = e; ++n;
         if (LB) { tr[n] = &ts; tp[n] = s_ts; x[n] = x1; ++n; }
         if (RANK) { tr[n] = &te; tp[n] = s_te; x[n] = x1; ++n; }
+        uint32_t tsz[N], r0[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) tsz[i] = tr[i]->size[tr[i]->nlv];
+        top_ranks<S, N>(tp, h, tsz, x, r0);
         int64_t j[N];
 #pragma unroll
-        for (int i = 0; i < N; ++i) j[i] = (int64_t)top_rank(tp[i], h, tr[i]->size[tr[i]->nlv], x[i]) - 1;
+        for (int i = 0; i < N; ++i) j[i] = (int64_t)r0[i] - 1;
         for (int l = ts.nlv - 1; l >= 0; --l) {
-            uint32_t b[N], sz[N], out[N];
+            uint32_t b[N], hi[N];
+            int64_t out[N];
 #pragma unroll
             for (int i = 0; i < N; ++i) {
                 v[i] = tr[i]->lv[l];
-                sz[i] = tr[i]->size[l];
                 b[i] = (uint32_t)(j[i] < 0 ? 0 : j[i]) * 64u;   // a search with no key <= x reads node 0, unused
+                hi[i] = min(b[i] + 63u, tr[i]->size[l] - 1u);
             }
+            node_last_le<S, N>(v, b, hi, x, out);
 #pragma unroll
-            for (int i = 0; i < N; ++i)
-                out[i] = (uint32_t)last_le_in(v[i], (int64_t)b[i], (int64_t)min(b[i] + 63u, sz[i] - 1u), x[i]);
-#pragma unroll
-            for (int i = 0; i < N; ++i) j[i] = (j[i] < 0) ? j[i] : (int64_t)out[i];
+            for (int i = 0; i < N; ++i) j[i] = (j[i] < 0) ? j[i] : out[i];
         }
         const uint32_t ub = (j[0] < 0) ? NONE : (uint32_t)j[0];
         pv.idx[k] = ub;
