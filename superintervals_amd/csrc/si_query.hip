@@ -1439,6 +1439,20 @@ __device__ __forceinline__ void node_last_le(const S* (&v)[N], const uint32_t (&
     for (int i = 0; i < N; ++i) out[i] = (p[i] == b[i] && !first_le[i]) ? (int64_t)b[i] - 1 : (int64_t)p[i];
 }
 
+// Last r in [lo, hi] with v[r] <= x, or lo - 1 (branchless halving over len elements: for
+// one key alone this reads no more than it must).
+template <typename S>
+__device__ __forceinline__ int64_t last_le_in(const S* __restrict__ v, int64_t lo, int64_t hi, S x) {
+    const S* base = v + lo;
+    uint32_t len = (uint32_t)(hi - lo + 1);
+    while (len > 1) {
+        const uint32_t half = len >> 1;
+        base = (base[half] <= x) ? base + half : base;
+        len -= half;
+    }
+    return (int64_t)(base - v) - ((*base <= x) ? 0 : 1);
+}
+
 // Last position with key <= x, or NONE, through the tree (s_top = its top level in LDS).
 template <typename S>
 __device__ __forceinline__ uint32_t ub_tree_pos(const UbTree<S>& t, const S* s_top, uint32_t h, S x) {
@@ -1449,12 +1463,8 @@ __device__ __forceinline__ uint32_t ub_tree_pos(const UbTree<S>& t, const S* s_t
     top_ranks<S, 1>(tp, h, tsz, xx, r);
     int64_t j = (int64_t)r[0] - 1;
     for (int l = t.nlv - 1; l >= 0 && j >= 0; --l) {
-        const S* v[1] = {t.lv[l]};
-        const uint32_t b[1] = {(uint32_t)j * 64u};
-        const uint32_t hi[1] = {min(b[0] + 63u, t.size[l] - 1u)};
-        int64_t o[1];
-        node_last_le<S, 1>(v, b, hi, xx, o);
-        j = o[0];
+        const int64_t b = j * 64, e = std::min<int64_t>(b + 63, (int64_t)t.size[l] - 1);
+        j = last_le_in(t.lv[l], b, e, x);
     }
     return (j < 0) ? NONE : (uint32_t)j;
 }
