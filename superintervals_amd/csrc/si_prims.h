@@ -221,8 +221,8 @@ int device_excl_scan(const TI* in, size_t n, TO* out, TO* total, TO* ws, hipStre
 // Radix sort: stable LSD over 64-bit keys with a 32-bit payload
 // ============================================================================
 constexpr int RS_THREADS = 256;
-// 2048-item tiles: the scatter's LDS (keys, payloads, digit counts: 30 KB) lets five
-// blocks share a CU; 4096-item tiles (54 KB, two blocks) were 10 % slower per pass
+// 2048-item tiles: the scatter's LDS (keys, payloads, digit starts: 25 KB) lets six
+// blocks share a CU; 4096-item tiles (two blocks) were 10 % slower per pass
 #ifndef SI_RS_ROUNDS
 #define SI_RS_ROUNDS 8
 #endif
@@ -266,74 +266,97 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_tile_hist(const uint64_t* __r
     th[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Stable scatter.  Each round ranks 256 items (4 waves x 64 lanes, in input order) by
-// matching digits with 8 ballots; ranks accumulate across rounds in LDS, the tile is
-// staged in LDS in digit order and written out so equal digits land contiguously.
-__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
-    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
-    uint32_t* __restrict__ vout, size_t n, int shift, const uint32_t* __restrict__ th_counts,
-    const uint32_t* __restrict__ th_base, uint32_t ntiles) {
-    __shared__ uint64_t s_key[RS_TILE];
-    __shared__ uint32_t s_val[RS_TILE];
-    __shared__ uint32_t s_wcnt[4][RS_BINS];
-    __shared__ uint32_t s_run[RS_BINS];
-    __shared__ uint32_t s_dstart[RS_BINS];
-    __shared__ uint32_t s_w[4];
-
+// Ranking of one tile of up to RS_TILE items held in registers (item r * 256 + tid in
+// key[r]): off[r] receives the item's position among the tile's items of equal digit, in
+// item order, and the return value is the tile's count of digit tid.  Each round of 256
+// items finds, per wave, the lanes with equal digits by 8 ballots; the wave's leader of
+// each digit records their number in s_c[round * 4 + wave][digit], and one column scan
+// per digit over the 32 (round, wave) entries turns those numbers into each entry's start.
+// s_c (RS_ROUNDS * 4 rows of RS_BINS 16-bit counts) must be free on entry; it is free
+// again once every thread has passed a barrier after the call.
+__device__ __forceinline__ uint32_t rs_rank_tile(const uint64_t (&key)[RS_ROUNDS], uint32_t tile_n, int shift,
+                                                 uint16_t (*s_c)[RS_BINS], uint32_t (&off)[RS_ROUNDS]) {
     const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
-    const uint32_t tile = blockIdx.x;
-    const size_t base = (size_t)tile * RS_TILE;
-    const uint32_t tile_n = (uint32_t)min((size_t)RS_TILE, n - base);
-
-    // digit starts within the tile = exclusive scan of this tile's digit counts
-    uint32_t my_cnt = th_counts[(size_t)tid * ntiles + tile];
-    uint32_t tot;
-    s_dstart[tid] = block_excl_scan256<uint32_t>(my_cnt, s_w, &tot);
-    s_run[tid] = 0;
+    uint32_t* const s_c32 = reinterpret_cast<uint32_t*>(&s_c[0][0]);
+#pragma unroll
+    for (int k = 0; k < RS_ROUNDS * 4 * RS_BINS / 2 / RS_THREADS; ++k) s_c32[k * RS_THREADS + tid] = 0u;
     __syncthreads();
-
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
     for (int r = 0; r < RS_ROUNDS; ++r) {
-        const uint32_t li = (uint32_t)r * RS_THREADS + tid;
-        const bool valid = li < tile_n;
-        uint64_t key = valid ? kin[base + li] : 0ull;
-        uint32_t val = valid ? vin[base + li] : 0u;
-        uint32_t digit = (uint32_t)(key >> shift) & 255u;
-        for (int k = lane; k < RS_BINS; k += WAVE) s_wcnt[w][k] = 0;
-        __syncthreads();
-        uint64_t peers = __ballot(valid);
+        const bool valid = (uint32_t)r * RS_THREADS + tid < tile_n;
+        const uint32_t digit = (uint32_t)(key[r] >> shift) & 255u;
+        uint64_t peers = __builtin_amdgcn_ballot_w64(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-            uint64_t bal = __ballot((digit >> b) & 1u);
+            const uint64_t bal = __builtin_amdgcn_ballot_w64((digit >> b) & 1u);
             peers &= ((digit >> b) & 1u) ? bal : ~bal;
         }
         const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-        if (valid && rank == 0) s_wcnt[w][digit] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        {   // per digit: prefix over the four waves, advance the running count
-            uint32_t c0 = s_wcnt[0][tid], c1 = s_wcnt[1][tid], c2 = s_wcnt[2][tid], c3 = s_wcnt[3][tid];
-            uint32_t run = s_run[tid];
-            s_wcnt[0][tid] = run;
-            s_wcnt[1][tid] = run + c0;
-            s_wcnt[2][tid] = run + c0 + c1;
-            s_wcnt[3][tid] = run + c0 + c1 + c2;
-            s_run[tid] = run + c0 + c1 + c2 + c3;
-        }
-        __syncthreads();
-        if (valid) {
-            uint32_t slot = s_dstart[digit] + s_wcnt[w][digit] + rank;
-            s_key[slot] = key;
-            s_val[slot] = val;
-        }
-        __syncthreads();
+        if (valid && rank == 0) s_c[r * 4 + w][digit] = (uint16_t)__popcll(peers);
+        off[r] = rank;
     }
+    __syncthreads();
+    uint32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < RS_ROUNDS * 4; ++k) {
+        const uint32_t c = s_c[k][tid];
+        s_c[k][tid] = (uint16_t)run;
+        run += c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) off[r] += s_c[r * 4 + w][(uint32_t)(key[r] >> shift) & 255u];
+    return run;
+}
+
+// Stable scatter of one tile (tile-major digit offsets from the scan of the per-tile digit
+// counts): the tile is ranked in registers, staged in LDS in digit order and written out so
+// equal digits land contiguously.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
+    uint32_t* __restrict__ vout, size_t n, int shift, const uint32_t* __restrict__ th_base, uint32_t ntiles) {
+    __shared__ union {
+        uint16_t c[RS_ROUNDS * 4][RS_BINS];
+        uint64_t key[RS_TILE];
+    } s_u;
+    __shared__ uint32_t s_val[RS_TILE];
+    __shared__ uint32_t s_dstart[RS_BINS];
+    __shared__ uint32_t s_w[4];
+
+    const int tid = threadIdx.x;
+    const uint32_t tile = blockIdx.x;
+    const size_t base = (size_t)tile * RS_TILE;
+    const uint32_t tile_n = (uint32_t)min((size_t)RS_TILE, n - base);
+    uint64_t key[RS_ROUNDS];
+    uint32_t val[RS_ROUNDS], off[RS_ROUNDS];
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const uint32_t li = (uint32_t)r * RS_THREADS + tid;
+        key[r] = li < tile_n ? kin[base + li] : 0ull;
+        val[r] = li < tile_n ? vin[base + li] : 0u;
+    }
+    const uint32_t cnt = rs_rank_tile(key, tile_n, shift, s_u.c, off);
+    uint32_t tot;
+    s_dstart[tid] = block_excl_scan256<uint32_t>(cnt, s_w, &tot);   // its barriers free s_u.c
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        if ((uint32_t)r * RS_THREADS + tid < tile_n) {
+            const uint32_t slot = s_dstart[(uint32_t)(key[r] >> shift) & 255u] + off[r];
+            s_u.key[slot] = key[r];
+            s_val[slot] = val[r];
+        }
+    }
+    __syncthreads();
+#pragma unroll
     for (int r = 0; r < RS_ROUNDS; ++r) {
         const uint32_t slot = (uint32_t)r * RS_THREADS + tid;
         if (slot < tile_n) {
-            uint64_t key = s_key[slot];
-            uint32_t digit = (uint32_t)(key >> shift) & 255u;
-            size_t dst = (size_t)th_base[(size_t)digit * ntiles + tile] + (slot - s_dstart[digit]);
-            kout[dst] = key;
+            const uint64_t k = s_u.key[slot];
+            const uint32_t digit = (uint32_t)(k >> shift) & 255u;
+            const size_t dst = (size_t)th_base[(size_t)digit * ntiles + tile] + (slot - s_dstart[digit]);
+            kout[dst] = k;
             vout[dst] = s_val[slot];
         }
     }
@@ -414,7 +437,7 @@ int radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, siz
         LAUNCH_CHECK();
         int rc = device_excl_scan<uint32_t, uint32_t>(tmp.th, nth, tmp.tb, nullptr, tmp.ws, st);
         if (rc != SI_OK) return rc;
-        k_rs_scatter<<<ntiles, RS_THREADS, 0, st>>>(ka, va, kb, vb, n, shift, tmp.th, tmp.tb, ntiles);
+        k_rs_scatter<<<ntiles, RS_THREADS, 0, st>>>(ka, va, kb, vb, n, shift, tmp.tb, ntiles);
         LAUNCH_CHECK();
         std::swap(ka, kb);
         std::swap(va, vb);

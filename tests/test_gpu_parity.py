@@ -465,6 +465,49 @@ def test_range_lengths_around_wave_written_threshold(sia, orc, dtype):
     ix.close()
 
 
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_plan_sort_forms_order_queries_by_end(sia, dtype):
+    """The plan sorts the queries by end, stably.  Both forms of its radix sort (plan_sync
+    1: a host read-back of the digit histograms skips the digits that are equal for every
+    query; 0: every digit is sorted, with no read-back) must give numpy's stable argsort of
+    the ends, for one item, one tile, a partial last tile and thousands of tiles, with ends
+    that are random, spread over the whole coordinate range, or few distinct values (long
+    runs of equal digits, and digits constant over the batch)."""
+    import torch
+    from superintervals_amd import autotune
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(91)
+    s, e = sia.workload_intervals(2, 100_000, seed=92, sorted=True)
+    ix = _dev_index(sia, s.astype(dtype), e.astype(dtype))
+    info = np.iinfo(dtype)
+    saved = autotune.current()
+    try:
+        for nq in (1, 2, 2047, 2048, 2049, 65_537, 3_000_001):
+            for kind in ("random", "wide", "few"):
+                print(f"[gpu] plan sort dtype={np.dtype(dtype).name} nq={nq} ends={kind}", flush=True)
+                if kind == "random":
+                    qe = rng.integers(0, 2_000_000, nq)
+                elif kind == "wide":
+                    qe = rng.integers(info.min // 2, info.max // 2, nq)
+                else:
+                    qe = rng.choice(np.array([-3, 0, 255, 256, 70_000], dtype=np.int64), nq)
+                qe = qe.astype(dtype)
+                qs = (qe.astype(np.int64) - rng.integers(0, 1000, nq)).astype(dtype)
+                want = np.argsort(qe, kind="stable")
+                tqs, tqe = torch.from_numpy(qs).to(dev), torch.from_numpy(qe).to(dev)
+                plan = torch.empty(ix.plan_bytes(nq), dtype=torch.uint8, device=dev)
+                for ps in (1, 0):
+                    autotune.apply({"plan_sync": ps, "plan_skip_low": 0})
+                    plan.fill_(0xA5)
+                    ix.plan_device(tqs, tqe, plan)
+                    torch.cuda.synchronize()
+                    q = plan[256:256 + 4 * nq].view(torch.int32).cpu().numpy()
+                    assert np.array_equal(q, want), (nq, kind, ps)
+    finally:
+        autotune.apply(saved)
+    ix.close()
+
+
 @pytest.mark.parametrize("dtype,long_len", [(np.int64, 0), (np.int64, 4096), (np.int32, 64), (np.int32, 1)])
 def test_long_list_walk(sia, orc, dtype, long_len):
     """The wave walk with a long list: hits below the first start >= s - len come from the
