@@ -471,8 +471,9 @@ def test_plan_sort_forms_order_queries_by_end(sia, dtype):
     1: a host read-back of the digit histograms skips the digits that are equal for every
     query; 0: every digit is sorted, with no read-back) must give numpy's stable argsort of
     the ends, for one item, one tile, a partial last tile and thousands of tiles, with ends
-    that are random, spread over the whole coordinate range, or few distinct values (long
-    runs of equal digits, and digits constant over the batch)."""
+    that are random, spread over the whole coordinate range, few distinct values (long runs
+    of equal digits, and digits constant over the batch) or all equal (the read-back form
+    then runs no pass and the sort's first-pass source is written out unsorted)."""
     import torch
     from superintervals_amd import autotune
     dev = torch.device("cuda:0")
@@ -483,14 +484,16 @@ def test_plan_sort_forms_order_queries_by_end(sia, dtype):
     saved = autotune.current()
     try:
         for nq in (1, 2, 2047, 2048, 2049, 65_537, 3_000_001):
-            for kind in ("random", "wide", "few"):
+            for kind in ("random", "wide", "few", "equal"):
                 print(f"[gpu] plan sort dtype={np.dtype(dtype).name} nq={nq} ends={kind}", flush=True)
                 if kind == "random":
                     qe = rng.integers(0, 2_000_000, nq)
                 elif kind == "wide":
                     qe = rng.integers(info.min // 2, info.max // 2, nq)
-                else:
+                elif kind == "few":
                     qe = rng.choice(np.array([-3, 0, 255, 256, 70_000], dtype=np.int64), nq)
+                else:   # every digit constant: the read-back form runs no pass
+                    qe = np.full(nq, 12_345, dtype=np.int64)
                 qe = qe.astype(dtype)
                 qs = (qe.astype(np.int64) - rng.integers(0, 1000, nq)).astype(dtype)
                 want = np.argsort(qe, kind="stable")
