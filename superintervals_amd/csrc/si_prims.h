@@ -230,20 +230,44 @@ constexpr int RS_ROUNDS = SI_RS_ROUNDS;   // rounds of 256 items per tile
 constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;   // items per tile
 constexpr int RS_BINS = 256;
 
-// Histograms of digits [dlo, dhi) at once (to skip constant-digit passes).
-__global__ __launch_bounds__(256) void k_rs_hist8(const uint64_t* __restrict__ keys, size_t n, int dlo, int dhi,
-                                                   uint32_t* __restrict__ hist8 /*[8][256]*/) {
+// Histograms of digits [dlo, dhi) at once (to skip constant-digit passes), tile by tile:
+// each thread loads its RS_ROUNDS keys of a tile before counting them.  A digit equal in
+// every lane of a wave (the case this pass looks for) is added once by one lane; others
+// by one LDS atomic per lane.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist8(const uint64_t* __restrict__ keys, size_t n, int dlo,
+                                                          int dhi, uint32_t* __restrict__ hist8 /*[8][256]*/) {
     __shared__ uint32_t h[8][RS_BINS];
-    for (int k = threadIdx.x; k < 8 * RS_BINS; k += blockDim.x) (&h[0][0])[k] = 0;
+    for (int k = threadIdx.x; k < 8 * RS_BINS; k += RS_THREADS) (&h[0][0])[k] = 0;
     __syncthreads();
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        uint64_t key = keys[i];
-        for (int d = dlo; d < dhi; ++d) atomicAdd(&h[d][(key >> (8 * d)) & 255], 1u);
+    const size_t ntiles = (n + RS_TILE - 1) / RS_TILE;
+    for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const size_t base = t * RS_TILE;
+        const uint32_t tile_n = (uint32_t)min((size_t)RS_TILE, n - base);
+        uint64_t key[RS_ROUNDS];
+#pragma unroll
+        for (int r = 0; r < RS_ROUNDS; ++r) {
+            const uint32_t li = (uint32_t)r * RS_THREADS + threadIdx.x;
+            key[r] = li < tile_n ? keys[base + li] : 0ull;
+        }
+#pragma unroll
+        for (int r = 0; r < RS_ROUNDS; ++r) {
+            const bool valid = (uint32_t)r * RS_THREADS + threadIdx.x < tile_n;
+            const uint64_t vm = __builtin_amdgcn_ballot_w64(valid);
+            if (vm == 0ull) continue;   // wave-uniform
+            for (int d = dlo; d < dhi; ++d) {
+                const uint32_t digit = (uint32_t)(key[r] >> (8 * d)) & 255u;
+                const uint32_t d0 = __builtin_amdgcn_readfirstlane(digit);   // lane 0's: valid whenever any is
+                if (__builtin_amdgcn_ballot_w64(valid && digit == d0) == vm) {
+                    if (lane_id() == 0) atomicAdd(&h[d][d0], (uint32_t)__popcll(vm));
+                } else if (valid) {
+                    atomicAdd(&h[d][digit], 1u);
+                }
+            }
+        }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < 8 * RS_BINS; k += blockDim.x) {
-        uint32_t c = (&h[0][0])[k];
+    for (int k = threadIdx.x; k < 8 * RS_BINS; k += RS_THREADS) {
+        const uint32_t c = (&h[0][0])[k];
         if (c) atomicAdd(&hist8[k], c);
     }
 }
@@ -448,8 +472,8 @@ int radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, siz
     if (skip_constant) {
         k_zero_u32<<<grid_for(8 * RS_BINS, 256), 256, 0, st>>>(tmp.hist8, 8 * RS_BINS);
         LAUNCH_CHECK();
-        unsigned hb = std::min<unsigned>(grid_for(n, 256 * 16), 2048);
-        k_rs_hist8<<<hb, 256, 0, st>>>(k0, n, digit_lo, digit_hi, tmp.hist8);
+        const unsigned hb = (unsigned)std::min<size_t>(rs_ntiles(n), 1024);
+        k_rs_hist8<<<hb, RS_THREADS, 0, st>>>(k0, n, digit_lo, digit_hi, tmp.hist8);
         LAUNCH_CHECK();
         HIP_TRY(hipMemcpyAsync(h8, tmp.hist8, sizeof(h8), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
