@@ -272,7 +272,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist8(const uint64_t* __restr
     }
 }
 
-// Per-tile digit counts, digit-major: th[d * ntiles + tile].
+// Per-tile digit counts, digit-major: th[d * ntiles + tile].  A digit equal across a wave
+// is added once, from one lane.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_tile_hist(const uint64_t* __restrict__ keys,
                                                              size_t n, int shift,
                                                              uint32_t* __restrict__ th,
@@ -283,8 +284,17 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_tile_hist(const uint64_t* __r
     size_t base = (size_t)blockIdx.x * RS_TILE;
 #pragma unroll 4
     for (int r = 0; r < RS_ROUNDS; ++r) {
-        size_t i = base + (size_t)r * RS_THREADS + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1u);
+        const size_t i = base + (size_t)r * RS_THREADS + threadIdx.x;
+        const bool valid = i < n;
+        const uint64_t vm = __builtin_amdgcn_ballot_w64(valid);
+        if (vm == 0ull) continue;   // wave-uniform
+        const uint32_t digit = valid ? (uint32_t)(keys[i] >> shift) & 255u : 0u;
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(digit);   // lane 0's: valid whenever any is
+        if (__builtin_amdgcn_ballot_w64(valid && digit == d0) == vm) {
+            if (lane_id() == 0) atomicAdd(&h[d0], (uint32_t)__popcll(vm));
+        } else if (valid) {
+            atomicAdd(&h[digit], 1u);
+        }
     }
     __syncthreads();
     th[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
