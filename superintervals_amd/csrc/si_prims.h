@@ -300,36 +300,6 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_tile_hist(const uint64_t* __r
     th[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Per digit (one block each), the exclusive scan of its per-tile counts over the tiles:
-// tb[d * ntiles + t] = counts of digit d in tiles before t, dtot[d] = its count over all.
-// The scatter adds the digit's start (the scan of dtot) itself.
-constexpr int RS_ROW_ITEMS = 16;   // consecutive tiles per thread in one round
-__global__ __launch_bounds__(RS_THREADS) void k_rs_row_scan(const uint32_t* __restrict__ th, uint32_t* __restrict__ tb,
-                                                            uint32_t ntiles, uint32_t* __restrict__ dtot) {
-    __shared__ uint32_t s_w[4];
-    const uint32_t* __restrict__ row = th + (size_t)blockIdx.x * ntiles;
-    uint32_t* __restrict__ out = tb + (size_t)blockIdx.x * ntiles;
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < ntiles; base += RS_THREADS * RS_ROW_ITEMS) {
-        const uint32_t i0 = base + threadIdx.x * RS_ROW_ITEMS;
-        uint32_t v[RS_ROW_ITEMS], sum = 0;
-#pragma unroll
-        for (int j = 0; j < RS_ROW_ITEMS; ++j) {
-            v[j] = (i0 + j < ntiles) ? row[i0 + j] : 0u;
-            sum += v[j];
-        }
-        uint32_t tot;
-        uint32_t run = carry + block_excl_scan256<uint32_t>(sum, s_w, &tot);
-#pragma unroll
-        for (int j = 0; j < RS_ROW_ITEMS; ++j) {
-            if (i0 + j < ntiles) out[i0 + j] = run;
-            run += v[j];
-        }
-        carry += tot;
-    }
-    if (threadIdx.x == 0) dtot[blockIdx.x] = carry;
-}
-
 // Ranking of one tile of up to RS_TILE items held in registers (item r * 256 + tid in
 // key[r]): off[r] receives the item's position among the tile's items of equal digit, in
 // item order, and the return value is the tile's count of digit tid.  Each round of 256
@@ -374,20 +344,18 @@ __device__ __forceinline__ uint32_t rs_rank_tile(const uint64_t (&key)[RS_ROUNDS
     return run;
 }
 
-// Stable scatter of one tile: the tile is ranked in registers, staged in LDS in digit order
-// and written out so equal digits land contiguously, digit d of tile t at the digit's start
-// (the exclusive scan of dtot) + the counts of d in tiles before t (row_pre).
+// Stable scatter of one tile (tile-major digit offsets from the scan of the per-tile digit
+// counts): the tile is ranked in registers, staged in LDS in digit order and written out so
+// equal digits land contiguously.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
-    uint32_t* __restrict__ vout, size_t n, int shift, const uint32_t* __restrict__ row_pre,
-    const uint32_t* __restrict__ dtot, uint32_t ntiles) {
+    uint32_t* __restrict__ vout, size_t n, int shift, const uint32_t* __restrict__ th_base, uint32_t ntiles) {
     __shared__ union {
         uint16_t c[RS_ROUNDS * 4][RS_BINS];
         uint64_t key[RS_TILE];
     } s_u;
     __shared__ uint32_t s_val[RS_TILE];
     __shared__ uint32_t s_dstart[RS_BINS];
-    __shared__ uint32_t s_gb[RS_BINS];
     __shared__ uint32_t s_w[4];
 
     const int tid = threadIdx.x;
@@ -405,8 +373,6 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     const uint32_t cnt = rs_rank_tile(key, tile_n, shift, s_u.c, off);
     uint32_t tot;
     s_dstart[tid] = block_excl_scan256<uint32_t>(cnt, s_w, &tot);   // its barriers free s_u.c
-    // where digit tid of this tile goes: the digit's start + its counts in earlier tiles
-    s_gb[tid] = block_excl_scan256<uint32_t>(dtot[tid], s_w, &tot) + row_pre[(size_t)tid * ntiles + tile];
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < RS_ROUNDS; ++r) {
@@ -423,7 +389,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
         if (slot < tile_n) {
             const uint64_t k = s_u.key[slot];
             const uint32_t digit = (uint32_t)(k >> shift) & 255u;
-            const size_t dst = (size_t)s_gb[digit] + (slot - s_dstart[digit]);
+            const size_t dst = (size_t)th_base[(size_t)digit * ntiles + tile] + (slot - s_dstart[digit]);
             kout[dst] = k;
             vout[dst] = s_val[slot];
         }
@@ -432,9 +398,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
 
 // Temporary device memory of a radix sort of n items (see rs_temp_bytes).
 struct RsTemp {
-    uint32_t* hist8;   // [8][256] digit histograms (in a pass: the digit totals)
+    uint32_t* hist8;   // [8][256] digit histograms
     uint32_t* th;      // per-tile digit counts, digit-major
-    uint32_t* tb;      // their exclusive scan along each digit's row
+    uint32_t* tb;      // their exclusive scan
+    uint32_t* ws;      // scan workspace
 };
 
 constexpr size_t rs_ntiles(size_t n) { return (n + RS_TILE - 1) / RS_TILE; }
@@ -443,7 +410,7 @@ constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 constexpr size_t rs_temp_bytes(size_t n) {
     const size_t nth = rs_ntiles(n) * RS_BINS;
-    return align256(8 * RS_BINS * 4) + 2 * align256(nth * 4);
+    return align256(8 * RS_BINS * 4) + 2 * align256(nth * 4) + align256(scan_ws_items(nth) * 4);
 }
 
 inline RsTemp rs_temp_carve(void* base, size_t n) {
@@ -452,7 +419,8 @@ inline RsTemp rs_temp_carve(void* base, size_t n) {
     RsTemp t;
     t.hist8 = (uint32_t*)p; p += align256(8 * RS_BINS * 4);
     t.th = (uint32_t*)p; p += align256(nth * 4);
-    t.tb = (uint32_t*)p;
+    t.tb = (uint32_t*)p; p += align256(nth * 4);
+    t.ws = (uint32_t*)p;
     return t;
 }
 
@@ -490,6 +458,7 @@ int radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, siz
     }
 
     const uint32_t ntiles = (uint32_t)rs_ntiles(n);
+    const size_t nth = (size_t)ntiles * RS_BINS;
     uint64_t *ka = k0, *kb = k1;
     uint32_t *va = v0, *vb = v1;
     for (int d = digit_lo; d < digit_hi; ++d) {
@@ -500,11 +469,9 @@ int radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, siz
         const int shift = 8 * d;
         k_rs_tile_hist<<<ntiles, RS_THREADS, 0, st>>>(ka, n, shift, tmp.th, ntiles);
         LAUNCH_CHECK();
-        // per-digit scans of the tile counts; the digit totals go where the histograms of the
-        // constant-digit check were (not read again)
-        k_rs_row_scan<<<RS_BINS, RS_THREADS, 0, st>>>(tmp.th, tmp.tb, ntiles, tmp.hist8);
-        LAUNCH_CHECK();
-        k_rs_scatter<<<ntiles, RS_THREADS, 0, st>>>(ka, va, kb, vb, n, shift, tmp.tb, tmp.hist8, ntiles);
+        int rc = device_excl_scan<uint32_t, uint32_t>(tmp.th, nth, tmp.tb, nullptr, tmp.ws, st);
+        if (rc != SI_OK) return rc;
+        k_rs_scatter<<<ntiles, RS_THREADS, 0, st>>>(ka, va, kb, vb, n, shift, tmp.tb, ntiles);
         LAUNCH_CHECK();
         std::swap(ka, kb);
         std::swap(va, vb);
