@@ -472,8 +472,9 @@ def test_plan_sort_forms_order_queries_by_end(sia, dtype):
     query; 0: every digit is sorted, with no read-back) must give numpy's stable argsort of
     the ends, for one item, one tile, a partial last tile and thousands of tiles, with ends
     that are random, spread over the whole coordinate range, few distinct values (long runs
-    of equal digits, and digits constant over the batch) or all equal (the read-back form
-    then runs no pass and the sort's first-pass source is written out unsorted)."""
+    of equal digits, and digits constant over the batch), runs of equal ends (digit counts
+    taken once per wave where a wave's digits agree, per lane elsewhere) or all equal (the
+    read-back form then runs no pass)."""
     import torch
     from superintervals_amd import autotune
     dev = torch.device("cuda:0")
@@ -484,7 +485,7 @@ def test_plan_sort_forms_order_queries_by_end(sia, dtype):
     saved = autotune.current()
     try:
         for nq in (1, 2, 2047, 2048, 2049, 65_537, 3_000_001):
-            for kind in ("random", "wide", "few", "equal"):
+            for kind in ("random", "wide", "few", "runs", "equal"):
                 print(f"[gpu] plan sort dtype={np.dtype(dtype).name} nq={nq} ends={kind}", flush=True)
                 if kind == "random":
                     qe = rng.integers(0, 2_000_000, nq)
@@ -492,6 +493,8 @@ def test_plan_sort_forms_order_queries_by_end(sia, dtype):
                     qe = rng.integers(info.min // 2, info.max // 2, nq)
                 elif kind == "few":
                     qe = rng.choice(np.array([-3, 0, 255, 256, 70_000], dtype=np.int64), nq)
+                elif kind == "runs":   # runs of ~100 equal ends: most waves see one digit value
+                    qe = np.repeat(rng.integers(-5_000_000, 5_000_000, nq // 100 + 1), 100)[:nq]
                 else:   # every digit constant: the read-back form runs no pass
                     qe = np.full(nq, 12_345, dtype=np.int64)
                 qe = qe.astype(dtype)
