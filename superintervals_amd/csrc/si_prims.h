@@ -344,12 +344,24 @@ __device__ __forceinline__ uint32_t rs_rank_tile(const uint64_t (&key)[RS_ROUNDS
     return run;
 }
 
+// Where a scatter writes item (key, val) at position dst: the sort's buffers, or, in a
+// sort's last pass, whatever arrays its caller wants the sorted items unpacked into.
+struct ArrayOut {
+    uint64_t* __restrict__ k;
+    uint32_t* __restrict__ v;
+    __device__ __forceinline__ void put(size_t dst, uint64_t key, uint32_t val) const {
+        k[dst] = key;
+        v[dst] = val;
+    }
+};
+
 // Stable scatter of one tile (tile-major digit offsets from the scan of the per-tile digit
 // counts): the tile is ranked in registers, staged in LDS in digit order and written out so
 // equal digits land contiguously.
+template <class Out>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
-    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
-    uint32_t* __restrict__ vout, size_t n, int shift, const uint32_t* __restrict__ th_base, uint32_t ntiles) {
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, Out out, size_t n, int shift,
+    const uint32_t* __restrict__ th_base, uint32_t ntiles) {
     __shared__ union {
         uint16_t c[RS_ROUNDS * 4][RS_BINS];
         uint64_t key[RS_TILE];
@@ -390,8 +402,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
             const uint64_t k = s_u.key[slot];
             const uint32_t digit = (uint32_t)(k >> shift) & 255u;
             const size_t dst = (size_t)th_base[(size_t)digit * ntiles + tile] + (slot - s_dstart[digit]);
-            kout[dst] = k;
-            vout[dst] = s_val[slot];
+            out.put(dst, k, s_val[slot]);
         }
     }
 }
@@ -440,9 +451,15 @@ __global__ void k_zero_u64(uint64_t* __restrict__ p, size_t n) {
 // digit in [digit_lo, digit_hi) with no read-back, so the sort can be captured in a graph.
 // tmp comes from rs_temp_carve.  On return *keys_sorted / *vals_sorted point at whichever
 // buffer holds the result.
+// With last (optional), the last pass run writes the sorted items through it instead of
+// into the buffers, and *last_written says whether one ran; only if none ran are the
+// items in *keys_sorted / *vals_sorted.
+template <class LastOut = ArrayOut>
 int radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, size_t n,
                      int digit_lo, int digit_hi, const RsTemp& tmp, uint64_t** keys_sorted,
-                     uint32_t** vals_sorted, hipStream_t st, bool skip_constant = true) {
+                     uint32_t** vals_sorted, hipStream_t st, bool skip_constant = true,
+                     const LastOut* last = nullptr, bool* last_written = nullptr) {
+    if (last_written) *last_written = false;
     *keys_sorted = k0;
     *vals_sorted = v0;
     if (n < 2) return SI_OK;
@@ -461,17 +478,28 @@ int radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, siz
     const size_t nth = (size_t)ntiles * RS_BINS;
     uint64_t *ka = k0, *kb = k1;
     uint32_t *va = v0, *vb = v1;
+    bool sort_digit[8] = {false};   // the digits sorted: all but those constant over the keys
+    int last_d = -1;
     for (int d = digit_lo; d < digit_hi; ++d) {
         bool constant = false;
         for (int b = 0; skip_constant && b < RS_BINS; ++b)
             if (h8[d * RS_BINS + b] == n) { constant = true; break; }
-        if (constant) continue;
+        sort_digit[d] = !constant;
+        if (!constant) last_d = d;
+    }
+    for (int d = digit_lo; d < digit_hi; ++d) {
+        if (!sort_digit[d]) continue;
         const int shift = 8 * d;
         k_rs_tile_hist<<<ntiles, RS_THREADS, 0, st>>>(ka, n, shift, tmp.th, ntiles);
         LAUNCH_CHECK();
         int rc = device_excl_scan<uint32_t, uint32_t>(tmp.th, nth, tmp.tb, nullptr, tmp.ws, st);
         if (rc != SI_OK) return rc;
-        k_rs_scatter<<<ntiles, RS_THREADS, 0, st>>>(ka, va, kb, vb, n, shift, tmp.tb, ntiles);
+        if (last && d == last_d) {
+            k_rs_scatter<LastOut><<<ntiles, RS_THREADS, 0, st>>>(ka, va, *last, n, shift, tmp.tb, ntiles);
+            if (last_written) *last_written = true;
+        } else {
+            k_rs_scatter<ArrayOut><<<ntiles, RS_THREADS, 0, st>>>(ka, va, ArrayOut{kb, vb}, n, shift, tmp.tb, ntiles);
+        }
         LAUNCH_CHECK();
         std::swap(ka, kb);
         std::swap(va, vb);

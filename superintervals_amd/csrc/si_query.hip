@@ -172,6 +172,40 @@ __global__ void k_plan_keys_i32(const int32_t* __restrict__ qs, const int32_t* _
     }
 }
 
+// The plan's sort written straight into the plan's arrays by its last pass (the unpacking
+// of k_plan_finish, done where the items are written).
+struct PlanOutI32 {
+    uint32_t* __restrict__ q;
+    int32_t* __restrict__ s;
+    int32_t* __restrict__ e;
+    __device__ __forceinline__ void put(size_t dst, uint64_t key, uint32_t val) const {
+        q[dst] = (uint32_t)key;
+        e[dst] = (int32_t)((uint32_t)(key >> 32) ^ 0x80000000u);
+        s[dst] = (int32_t)val;
+    }
+};
+struct PlanOutI64 {
+    uint32_t* __restrict__ q;
+    int64_t* __restrict__ s;
+    int64_t* __restrict__ e;
+    const int64_t* __restrict__ qs;
+    __device__ __forceinline__ void put(size_t dst, uint64_t key, uint32_t val) const {
+        q[dst] = val;
+        e[dst] = (int64_t)(key ^ 0x8000000000000000ull);
+        s[dst] = qs[val];
+    }
+};
+
+// The plan's flags, when its last sort pass wrote the arrays (k_plan_finish sets them otherwise).
+template <typename S>
+__global__ void k_plan_counters(PlanView<S> pv, bool with_lb, bool with_rank) {
+    if (threadIdx.x == 0) {
+        pv.counter[0] = 0;
+        pv.counter[4] = with_lb ? 1u : 0u;
+        pv.counter[5] = with_rank ? 1u : 0u;
+    }
+}
+
 // int64: key = biased end, payload = query number.
 __global__ void k_plan_keys_i64(const int64_t* __restrict__ qe, size_t nq, uint64_t* __restrict__ key,
                                 uint32_t* __restrict__ val) {
@@ -342,19 +376,6 @@ int plan_impl(const siIndex* ix, const S* qs, const S* qe, size_t nq, void* plan
     uint32_t* v1 = (uint32_t*)(p + L.off_v1);
     RsTemp rt = rs_temp_carve(p + L.off_rs, nq);
     const unsigned g = grid_for(nq, Q_THREADS);
-    uint64_t* ks;
-    uint32_t* vs;
-    int rc;
-    if (sizeof(S) == 4) {
-        k_plan_keys_i32<<<g, Q_THREADS, 0, st>>>((const int32_t*)qs, (const int32_t*)qe, nq, k0, v0);
-        LAUNCH_CHECK();
-        rc = radix_sort_pairs(k0, v0, k1, v1, nq, 4 + g_tune.plan_skip_low, 8, rt, &ks, &vs, st, read_back);
-    } else {
-        k_plan_keys_i64<<<g, Q_THREADS, 0, st>>>((const int64_t*)qe, nq, k0, v0);
-        LAUNCH_CHECK();
-        rc = radix_sort_pairs(k0, v0, k1, v1, nq, g_tune.plan_skip_low, 8, rt, &ks, &vs, st, read_back);
-    }
-    if (rc != SI_OK) return rc;
     // upper_bound of the sorted ends (and lower_bound of the starts when the walks take the
     // guaranteed run): through the search tree when the index has one
     const bool learned = g_tune.ub_learned && ix->learned;
@@ -365,7 +386,26 @@ int plan_impl(const siIndex* ix, const S* qs, const S* qe, size_t nq, void* plan
     const bool with_rank = tree && rank_exact(ix) && ix->eb.d_top && ix->eb.nlv == ix->ub.nlv &&
                            ix->eb.top_h == ix->ub.top_h;
     const PlanView<S> pv = plan_view<S>(plan, nq);
-    k_plan_finish<S><<<g, Q_THREADS, 0, st>>>(ix->view<S>(), ks, vs, qs, nq, pv, !tree && !learned, with_lb, with_rank);
+    uint64_t* ks;
+    uint32_t* vs;
+    int rc;
+    bool unpacked = false;   // the sort's last pass wrote q, s, e (when the bounds come from a search below)
+    if (sizeof(S) == 4) {
+        k_plan_keys_i32<<<g, Q_THREADS, 0, st>>>((const int32_t*)qs, (const int32_t*)qe, nq, k0, v0);
+        LAUNCH_CHECK();
+        const PlanOutI32 out{pv.q, (int32_t*)pv.s, (int32_t*)pv.e};
+        rc = radix_sort_pairs(k0, v0, k1, v1, nq, 4 + g_tune.plan_skip_low, 8, rt, &ks, &vs, st, read_back,
+                              (tree || learned) ? &out : nullptr, &unpacked);
+    } else {
+        k_plan_keys_i64<<<g, Q_THREADS, 0, st>>>((const int64_t*)qe, nq, k0, v0);
+        LAUNCH_CHECK();
+        const PlanOutI64 out{pv.q, (int64_t*)pv.s, (int64_t*)pv.e, (const int64_t*)qs};
+        rc = radix_sort_pairs(k0, v0, k1, v1, nq, g_tune.plan_skip_low, 8, rt, &ks, &vs, st, read_back,
+                              (tree || learned) ? &out : nullptr, &unpacked);
+    }
+    if (rc != SI_OK) return rc;
+    if (unpacked) k_plan_counters<S><<<1, 64, 0, st>>>(pv, with_lb, with_rank);
+    else k_plan_finish<S><<<g, Q_THREADS, 0, st>>>(ix->view<S>(), ks, vs, qs, nq, pv, !tree && !learned, with_lb, with_rank);
     LAUNCH_CHECK();
     if (learned) return si_learned_plan_bounds(ix, pv.e, pv.s, nq, pv.idx, with_lb ? pv.lb : nullptr, st);
     if (tree) return launch_plan_bounds<S>(ix, pv, nq, with_lb, with_rank, st);
