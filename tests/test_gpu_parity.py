@@ -1060,11 +1060,28 @@ def test_cpp_api_program(sia):
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
 
 
-def test_full_size_lognormal_10Mx10M_properties(sia, orc):
+@pytest.mark.parametrize("knobs", [None, {"tile": 512, "count_wgs": 0, "fill_wgs": 0, "lane_steps": 1024,
+                                          "count_rank": 1, "plan_sync": 0, "plan_skip_low": 1}],
+                         ids=["defaults", "benched"])
+def test_full_size_lognormal_10Mx10M_properties(sia, orc, knobs):
     """10M intervals x 10M queries (log-normal lengths): size-independent properties of the whole
-    output plus an oracle check of a query sample."""
+    output plus an oracle check of a query sample, with the library's default launch choices
+    and with those the headline benchmark's autotuner picks (512-query tiles, no plan
+    read-back, the lowest end digit unsorted)."""
     import torch
-    print("[gpu] full-size log-normal 10M x 10M", flush=True)
+    from superintervals_amd import autotune
+    print(f"[gpu] full-size log-normal 10M x 10M knobs={knobs}", flush=True)
+    saved = autotune.current()
+    if knobs:
+        autotune.apply(knobs)
+    try:
+        _full_size_lognormal(sia, orc)
+    finally:
+        autotune.apply(saved)
+
+
+def _full_size_lognormal(sia, orc):
+    import torch
     n = nq = 10_000_000
     s, e = sia.workload_intervals(2, n, seed=1, sorted=True)
     qs, qe = sia.workload_queries(2, nq, seed=2)
